@@ -1,0 +1,110 @@
+/*
+ * intT.h -- reconstruction of the reference's missing configuration header.
+ *
+ * The reference includes "intT.h" from every source file (intEV.cu:7,
+ * Integrand.c:7, Tint.c:11, EffAct.c:5, periodic.c:11, testT.c:5) but does
+ * not ship it.  Its contract, recovered from use, is: `struct Wparams`, the
+ * profile selector macro `profile` with enumerator macros usable in `#if`,
+ * the scalars tubedist, lmin, q1, q2, Nspline, pi and verbosity, and the
+ * prototypes of integrateT, Tfunc, getwl, EV and EVMean.
+ *
+ * Against this header the reference's C drivers (Tint.c, Integrand.c,
+ * EffAct.c) link to libwfl.so unchanged except for the CUDA includes; the
+ * compile-time choices are pushed into the library with wfl_sync_intT().
+ */
+#ifndef WFL_INTT_H
+#define WFL_INTT_H
+#include <hip/hip_runtime_api.h>   /* float4 for C and C++ */
+#include "wfl.h"
+
+#define step 0
+#define smooth 1
+#define quadratic 2
+#define gaussian 3
+#define periodic 4
+#define spline 5
+#define fixflux 6
+
+#ifndef profile
+#define profile gaussian
+#endif
+#ifndef tubedist
+#define tubedist 1.0
+#endif
+#ifndef lmin
+#define lmin 0.1
+#endif
+#ifndef q1
+#define q1 0.443993816168079
+#endif
+#ifndef q2
+#define q2 0.0742477533879612
+#endif
+#ifndef Nspline
+#define Nspline 512
+#endif
+#ifndef pi
+#define pi 3.14159265358979323846
+#endif
+#ifndef verbosity
+#define verbosity 1
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Field order and types as EffAct.c:399-413, 454-455 and intEV.cu:115-137 use
+ * them.  Device pointers: worldlines (float4 AoS [Nl*Nppl]), Wsscal_d,
+ * Wsferm_d (float4 [Nl]), flcoefs (float4 [Nspline]), rho2sp (float
+ * [Nspline]).  Host pointers: Wsscal_h, Wsferm_h (float4 [Nl]). */
+struct Wparams {
+  int Nl, Nppl;
+  double F, l2;
+  float4 xcm;
+  float4 *Wsscal_h, *Wsscal_d, *Wsferm_h, *Wsferm_d, *worldlines;
+  double SE;
+  int nBlocks, nThreads, ng, fermion;
+  float4 *flcoefs;
+  float *rho2sp;
+};
+
+/* intEV.cu:111 -- evaluates every loop at (params->xcm, sqrt(T)) on the GPU,
+ * overwrites Wsscal_h/Wsferm_h for all Nl loops and returns EVMean over the
+ * 128 loops of WLlist. */
+double EV(double T, void *p, int *WLlist);
+/* Integrand.c:325 */
+void EVMean(double *EV, float4 *Wsscal_h, float4 *Wsferm_h, int n, int *WL, double T, int fermion);
+/* Integrand.c:195 -- returns malloc'd double[ng] */
+double *Tfunc(double T, void *p, int *order);
+/* Tint.c:100 -- returns malloc'd double[ng] */
+double *integrateT(int n, void *p);
+/* getwl.c:6 -- text loops into the reference's float4 AoS layout */
+int getwl(float4 *worldlines, char *filename, int Nl, int Nppl, int Dim);
+/* Integrand.c:20-172 closed forms */
+double Exact(double T, double B, int fermion);
+double MagField(double rho2, void *p);
+double smTscal(double B, double T);
+double smTferm(double B, double T);
+
+/* Push this translation unit's compile-time profile choice into the library
+ * (call once before the first EV()). */
+static inline int wfl_sync_intT(const struct Wparams *p) {
+  wfl_geometry g;
+  g.kind = profile;
+  g.F = p->F;
+  g.l2 = p->l2;
+  g.a = tubedist;
+  g.lam_min = lmin;
+  g.fix_q1 = q1;
+  g.fix_q2 = q2;
+  g.lattice = 0.0;
+  g.images = 0;
+  g.spline_table = (profile == spline) ? 0 : -1;
+  return wfl_set_profile(&g, Nspline);
+}
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,182 @@
+/*
+ * wfl.h -- C ABI of the MI355X worldline Wilson-loop engine (libwfl.so).
+ *
+ * Plain C: pointers, sizes and status codes only.  Device buffers are
+ * ordinary HIP device pointers passed as void*; streams are hipStream_t
+ * passed as void* (NULL = the default stream).
+ *
+ * Two entry levels:
+ *   1. The reference's own call, `double EV(double T, void *p, int *WLlist)`
+ *      (declared in intT.h, which reconstructs the reference's missing
+ *      header), together with EVMean, Tfunc, integrateT and the profile
+ *      closed forms.
+ *   2. The batched engine below: many (geometry, x_cm, T) tuples over one
+ *      device-resident loop ensemble in one launch sequence, reduced on the
+ *      device to FP64 per-group sums.
+ *
+ * Status codes: WFL_OK (0) on success, negative on error; wfl_last_error()
+ * returns a message for the calling thread's most recent failure.
+ */
+#ifndef WFL_H
+#define WFL_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WFL_OK 0
+#define WFL_ERR_ARG (-1)      /* invalid argument or shape */
+#define WFL_ERR_DEVICE (-2)   /* HIP runtime error (no device, launch failure) */
+#define WFL_ERR_NOMEM (-3)    /* host or device allocation failed */
+#define WFL_ERR_STATE (-4)    /* call out of order (e.g. run before create) */
+
+/* Field profiles: the cases of flambda/fplambda (intEV.cu:261-406).
+ * WFL_FIXFLUX_FF is the intEVff.cu variant (phi threshold 1, chi fit B). */
+enum { WFL_STEP = 0, WFL_SMOOTH = 1, WFL_QUADRATIC = 2, WFL_GAUSSIAN = 3,
+       WFL_PERIODIC = 4, WFL_SPLINE = 5, WFL_FIXFLUX = 6, WFL_FIXFLUX_FF = 7 };
+
+/* Numerical definition of the segment integral.
+ * REFERENCE: Idt exactly as intEV.cu:409-459 defines it, including the clamps
+ *            of A, B, C and rho^2 (the linear coefficient B = z.(z'-z) is
+ *            clamped to >= 1e-8) and both endpoints evaluated per segment.
+ * PHYSICAL:  the same composite Simpson rule along the true path
+ *            rho^2(t) = |z + t(z'-z)|^2 with no clamps; the removable
+ *            singularity of f(u)/u at u = 0 is taken analytically and each
+ *            point's endpoint value is shared by its two segments. */
+enum { WFL_MODE_REFERENCE = 0, WFL_MODE_PHYSICAL = 1 };
+
+/* Arithmetic of the segment phases and Simpson sums.
+ * FP64:  everything in double.
+ * MIXED: segment geometry and profile evaluations in FP32, per-loop sums
+ *        carried in FP64 blocks, W-1 and the ensemble sums in FP64. */
+enum { WFL_FP64 = 0, WFL_MIXED = 1 };
+
+/* One flux-tube geometry (a field profile, optionally repeated on a square
+ * lattice).  F and l2 are quantized to FP32 by the engine so that both
+ * precisions evaluate the same problem (the reference narrows them to
+ * float at intEV.cu:124). */
+typedef struct {
+  int kind;             /* WFL_* profile */
+  double F;             /* flux parameter (Integrand.c:137: eB = 2 F f'(rho^2)) */
+  double l2;            /* lambda^2 */
+  double a;             /* tubedist: ring/lattice scale of periodic, spline, fixflux */
+  double lam_min;       /* lmin (fixflux) */
+  double fix_q1, fix_q2;/* q1, q2 (fixflux normalisations) */
+  double lattice;       /* square-lattice spacing of image tubes; <= 0: isolated */
+  int images;           /* image window half-width M: (2M+1)^2 tubes around x_cm's cell */
+  int spline_table;     /* index of the spline table (WFL_SPLINE), else -1 */
+} wfl_geometry;
+
+/* A clamped cubic spline of f(rho^2) on sorted knots (periodic.c:45-110
+ * layout: knots as float, coefficients as float4 (a, b, c, d)). */
+typedef struct {
+  int n;
+  const float *rho2;    /* host [n] */
+  const float *coefs;   /* host [n][4] */
+} wfl_spline_table;
+
+/* Batched workload description (host arrays; copied by wfl_plan_create). */
+typedef struct {
+  int mode;             /* WFL_MODE_* */
+  int precision;        /* WFL_FP64 or WFL_MIXED */
+  int nsub;             /* Simpson sub-intervals per segment (even, >= 2) */
+  int spinor;           /* 0: scalar QED; 1: spinor QED (spin factor) */
+  int group;            /* loops per group (the reference uses 128) */
+  int ngeom;
+  const wfl_geometry *geoms;
+  int nspline;
+  const wfl_spline_table *splines;
+  int ntup;             /* evaluation tuples */
+  const int *tup_geom;  /* [ntup] geometry index */
+  const double *tup_cx; /* [ntup] x_cm components */
+  const double *tup_cy;
+  const double *tup_T;  /* [ntup] proper time T (quantized: sqrt(T) to FP32) */
+  const int *geom_nloops; /* [ngeom] loops used per geometry (NULL: all Nl) */
+  int want_perloop;     /* also keep per-loop (scal[3], ferm[3]) FP64 outputs */
+  int mapping;          /* 0: autotune; 1: lane per tuple; 2: wave per loop */
+} wfl_batch;
+
+typedef struct wfl_loops_s wfl_loops;   /* device-resident loop ensemble */
+typedef struct wfl_plan_s wfl_plan;     /* uploaded batched workload */
+
+/* ---- device and errors ---- */
+const char *wfl_last_error(void);
+int wfl_device_count(int *count);
+int wfl_device_sync(void);
+
+/* ---- loop ensembles ----
+ * Host SoA, loop-major: x[l*N + i], y[l*N + i]; N must be a multiple of 4. */
+int wfl_loops_create(wfl_loops **out, const float *x, const float *y, int Nl, int N);
+/* From the reference's device layout: float4 AoS [Nl*N] (getwl.c:18-35). */
+int wfl_loops_from_aos_device(wfl_loops **out, const void *dev_float4, int Nl, int N);
+/* Seeded unit loops generated on the host (Philox4x32-10, key
+ * 0x5EED0000 + config_id) and uploaded; bit-identical to wfl_gen_loops. */
+int wfl_loops_generate(wfl_loops **out, uint32_t config_id, int Nl, int N);
+int wfl_loops_shape(const wfl_loops *L, int *Nl, int *N);
+void wfl_loops_destroy(wfl_loops *L);
+
+/* Host generator used by wfl_loops_generate (x, y: [Nl*N]). */
+int wfl_gen_loops(uint32_t config_id, int Nl, int N, float *x, float *y);
+/* Text reader for the reference's worldline files (getwl.c:6-40): per loop
+ * N x values then N y values.  Writes SoA x, y. */
+int wfl_read_text_loops(const char *path, int Nl, int N, float *x, float *y);
+
+/* ---- batched engine ---- */
+int wfl_plan_create(wfl_plan **out, const wfl_batch *b, wfl_loops *L);
+/* Enqueue the whole evaluation on `stream` (asynchronous). */
+int wfl_plan_run(wfl_plan *p, void *stream);
+/* Capture one wfl_plan_run into a hipGraph and replay it (T-sweep reuse). */
+int wfl_plan_graph_run(wfl_plan *p, void *stream);
+/* Copy results to the host (synchronizes): gsum [ntup][ngroups][3] sums of
+ * (W_N - 1, W_{N/2} - 1, W_{N/4} - 1); perloop [ntup][Nl][6] if kept. */
+int wfl_plan_read(wfl_plan *p, double *gsum, double *perloop);
+int wfl_plan_ngroups(const wfl_plan *p);
+/* Device pointer of the group sums (for callers that keep results on device). */
+const double *wfl_plan_gsum_device(const wfl_plan *p);
+/* Number of non-finite per-loop values met in the last run (diagnostic). */
+int wfl_plan_nonfinite(wfl_plan *p, long long *count);
+/* Mapping the autotuner chose (1: lane per tuple, 2: wave per loop). */
+int wfl_plan_mapping(const wfl_plan *p);
+void wfl_plan_destroy(wfl_plan *p);
+/* create + run + read + destroy. */
+int wfl_eval_batch(const wfl_batch *b, wfl_loops *L, double *gsum, double *perloop);
+
+/* Richardson-extrapolated <W> - 1 of one group from its sums
+ * (Integrand.c:400): 8/3 m_N - 2 m_{N/2} + 1/3 m_{N/4}. */
+double wfl_group_ev_minus1(const double gsum3[3], int count);
+
+/* ---- spline set-up (periodic.c:45-110), FP64 ----
+ * quirk = 1 reproduces the reference's float round trip of c. */
+int wfl_spline_build(const double *fli, const float *rho2, int n, int quirk, float *coefs4);
+
+/* ---- drop-in configuration for EV() (what intT.h fixes at compile time) ---- */
+int wfl_set_profile(const wfl_geometry *g, int nspline);
+int wfl_set_numerics(int mode, int precision, int nsub);
+/* Forget the cached per-loop results and loop copies held for EV(). */
+void wfl_ev_reset(void);
+/* Kernel launches EV() made since the last reset (cache diagnostics). */
+long long wfl_ev_launches(void);
+
+/* ---- proper-time integration on a log-spaced grid ----
+ * Composite Simpson in ln T (3/8 rule on the last three intervals when the
+ * interval count is odd). */
+double wfl_logsimpson(const double *T, const double *f, int n);
+/* The proper time the kernels evaluate for a requested T:
+ * ((double)(float)sqrt(T))^2 (sqrt(T) is narrowed to FP32 as at intEV.cu:124). */
+double wfl_effective_T(double T);
+/* L_eff(rho) per loop group from batched sums on a log T grid.
+ * gsum [nT][ngroups][3] holds the group sums of the nT tuples of one
+ * (geometry, x_cm) in T order, counts [ngroups] the loops per group, T the
+ * effective proper times.  policy 1 applies the reference's analytic small-
+ * and large-T branches (Integrand.c:252-288), policy 0 uses the worldline
+ * estimate at every node.  Normalised as Tint.c:69-72: /(-2 pi) scalar,
+ * /(4 pi) spinor. */
+int wfl_leff_groups(const wfl_geometry *g, int fermion, int nT, const double *T, double cx, double cy,
+                    const double *gsum, const int *counts, int ngroups, int policy, double *leff);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

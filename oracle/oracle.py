@@ -1,0 +1,200 @@
+"""ctypes binding of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker, never by the product.
+See wfl_oracle.h for the reference file:line each function restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+STEP, SMOOTH, QUADRATIC, GAUSSIAN, PERIODIC, SPLINE, FIXFLUX, FIXFLUX_FF = range(8)
+MODE_REFERENCE, MODE_PHYSICAL = 0, 1
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+if not os.path.exists(LIB_PATH):
+    build()
+
+
+class OrcGeom(C.Structure):
+    _fields_ = [
+        ("profile", C.c_int), ("l2", C.c_double), ("F", C.c_double),
+        ("tubedist", C.c_double), ("lmin", C.c_double), ("q1", C.c_double), ("q2", C.c_double),
+        ("nspline", C.c_int), ("rho2sp", C.c_void_p), ("coefs", C.c_void_p),
+        ("lattice", C.c_double), ("images", C.c_int),
+    ]
+
+
+lib = C.CDLL(LIB_PATH)
+_P, _I, _D = C.c_void_p, C.c_int, C.c_double
+for _n, _r, _a in [
+    ("orc_philox4x32_10", None, [_P, _P, _P]),
+    ("orc_gen_loops", None, [C.c_uint32, _I, _I, _P, _P]),
+    ("orc_f", _D, [_P, _D]),
+    ("orc_fp", _D, [_P, _D]),
+    ("orc_expint", _D, [_D]),
+    ("orc_interp", _D, [_D, _P, _P, _I]),
+    ("orc_wilson_loop", None, [_P, _I, _I, _I, _P, _P, _I, _D, _D, _D, _P, _P]),
+    ("orc_eval_batch", None, [_P, _I, _I, _I, _P, _P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I]),
+    ("orc_wminus1", None, [_I, _I, _P, _P, _P]),
+    ("orc_evmean", _D, [_P, _P, _I, _P, _I]),
+    ("orc_exact", _D, [_D, _D, _I]),
+    ("orc_smTscal", _D, [_D, _D]),
+    ("orc_smTferm", _D, [_D, _D]),
+    ("orc_magfield", _D, [_P, _D]),
+    ("orc_smallT", _D, [_P, _D, _D]),
+    ("orc_igrand", _D, [_P, _I, _D, _D, _D, _D, _P]),
+    ("orc_spline", None, [_P, _P, _I, _I, _P, _P]),
+    ("orc_logsimpson", _D, [_P, _P, _I]),
+]:
+    _f = getattr(lib, _n)
+    _f.restype = _r
+    _f.argtypes = _a
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def q32(v):
+    """FP32 quantization the engine applies to every kernel parameter."""
+    return float(np.float32(v))
+
+
+class Geom:
+    """Oracle geometry; parameters are quantized to FP32 as the engine does."""
+
+    def __init__(self, profile=GAUSSIAN, F=1.0, l2=1.0, tubedist=1.0, lmin=0.1,
+                 q1=0.443993816168079, q2=0.0742477533879612, lattice=0.0, images=0,
+                 knots=None, coefs=None, quantize=True):
+        qz = q32 if quantize else float
+        self.knots = None if knots is None else np.ascontiguousarray(knots, np.float32)
+        self.coefs = None if coefs is None else np.ascontiguousarray(coefs, np.float32)
+        self.c = OrcGeom(profile, qz(l2), qz(F), qz(tubedist), qz(lmin), qz(q1), qz(q2),
+                         0 if knots is None else len(self.knots), _p(self.knots), _p(self.coefs),
+                         qz(lattice) if lattice > 0 else 0.0, images)
+
+    @property
+    def ref(self):
+        return C.byref(self.c)
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib.orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def gen_loops(config_id, Nl, N):
+    x = np.empty((Nl, N), np.float32)
+    y = np.empty((Nl, N), np.float32)
+    lib.orc_gen_loops(config_id, Nl, N, _p(x), _p(y))
+    return x, y
+
+
+def wilson_loop(geom, mode, nsub, spinor, wx, wy, cx, cy, rtT):
+    wx = np.ascontiguousarray(wx, np.float32)
+    wy = np.ascontiguousarray(wy, np.float32)
+    sc = np.zeros(3)
+    fe = np.zeros(3)
+    lib.orc_wilson_loop(geom.ref, mode, nsub, spinor, _p(wx), _p(wy), len(wx), float(cx), float(cy),
+                        float(rtT), _p(sc), _p(fe))
+    return sc, fe
+
+
+def eval_batch(geoms, mode, nsub, spinor, x, y, group, tup_geom, tup_cx, tup_cy, tup_T,
+               perloop=True, nthreads=0):
+    """Same tuple semantics as the engine: cx, cy quantized to FP32 and
+    sqrt(T) narrowed to FP32.  Returns (gsum [ntup, ng, 3], perloop [ntup, Nl, 6])."""
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    Nl, N = x.shape
+    ntup = len(tup_geom)
+    ng = (Nl + group - 1) // group
+    garr = (OrcGeom * len(geoms))(*[g.c for g in geoms])
+    tg = np.ascontiguousarray(tup_geom, np.int32)
+    cx = np.array([q32(v) for v in tup_cx], np.float64)
+    cy = np.array([q32(v) for v in tup_cy], np.float64)
+    rt = np.array([q32(np.sqrt(t)) for t in tup_T], np.float64)
+    gs = np.zeros((ntup, ng, 3))
+    pl = np.zeros((ntup, Nl, 6)) if perloop else None
+    lib.orc_eval_batch(C.cast(garr, C.c_void_p), mode, nsub, spinor, _p(x), _p(y), Nl, N, group, ntup,
+                       _p(tg), _p(cx), _p(cy), _p(rt), _p(pl), _p(gs), nthreads)
+    return gs, pl
+
+
+def wminus1(mode, spinor, sc, fe):
+    sc = np.ascontiguousarray(sc, np.float64)
+    fe = np.ascontiguousarray(fe, np.float64)
+    o = np.zeros(3)
+    lib.orc_wminus1(mode, spinor, _p(sc), _p(fe), _p(o))
+    return o
+
+
+def evmean(ws4, wf4, WL, fermion):
+    ws4 = np.ascontiguousarray(ws4, np.float32)
+    wf4 = np.ascontiguousarray(wf4 if wf4 is not None else np.zeros_like(ws4), np.float32)
+    WL = np.ascontiguousarray(WL, np.int32)
+    return lib.orc_evmean(_p(ws4), _p(wf4), len(WL), _p(WL), fermion)
+
+
+def igrand(geom, fermion, T, rho, rho2, EV):
+    b = C.c_int(-1)
+    v = lib.orc_igrand(geom.ref, fermion, T, rho, rho2, EV, C.byref(b))
+    return v, b.value
+
+
+def spline(fli, rho2, quirk=False):
+    fli = np.ascontiguousarray(fli, np.float64)
+    rho2 = np.ascontiguousarray(rho2, np.float32)
+    n = len(rho2)
+    c4 = np.zeros((n, 4), np.float32)
+    cd = np.zeros((n, 4), np.float64)
+    lib.orc_spline(_p(fli), _p(rho2), n, int(quirk), _p(c4), _p(cd))
+    return c4, cd
+
+
+def logsimpson(T, f):
+    T = np.ascontiguousarray(T, np.float64)
+    f = np.ascontiguousarray(f, np.float64)
+    return lib.orc_logsimpson(_p(T), _p(f), len(T))
+
+
+def leff_groups(geom, fermion, T_eff, cx, cy, gsum, counts, policy=1):
+    """Oracle L_eff per group (Tfunc policy + log-Simpson), /(-2pi) or /(4pi)."""
+    rho2 = cx * cx + cy * cy
+    rho = np.sqrt(rho2)
+    ng = gsum.shape[1]
+    out = np.zeros(ng)
+    for q in range(ng):
+        f = np.zeros(len(T_eff))
+        for k, T in enumerate(T_eff):
+            g3 = gsum[k, q]
+            evm1 = (8.0 / 3.0 * g3[0] - 2.0 * g3[1] + 1.0 / 3.0 * g3[2]) / counts[q]
+            if policy:
+                v, br = igrand(geom, fermion, T, rho, rho2, 1.0 + evm1)
+                if br == 3:
+                    B = lib.orc_magfield(geom.ref, rho2)
+                    TB = 0.30282212 * T * B
+                    v = np.exp(-T) / T**3 * (evm1 - TB * TB / 3.0 if fermion else evm1 + TB * TB / 6.0)
+            else:
+                B = lib.orc_magfield(geom.ref, rho2)
+                TB = 0.30282212 * T * B
+                v = np.exp(-T) / T**3 * (evm1 - TB * TB / 3.0 if fermion else evm1 + TB * TB / 6.0)
+            f[k] = v
+        I = logsimpson(T_eff, f)
+        out[q] = I / (4 * np.pi) if fermion else I / (-2 * np.pi)
+    return out

@@ -1,0 +1,137 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the FP64 oracle.
+
+Bars (per-loop sub-loop phases and spin sums, and per-group sums of W - 1):
+  FP64 kernels   relative error <= 1e-10 of the largest magnitude in the case
+  mixed kernels  relative error <= 2e-5 per loop and <= 1e-5 in group sums
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NL, N = 96, 256
+
+
+@pytest.fixture(scope="module")
+def loops(gpu):
+    x, y = gpu.gen_loops(3, NL, N)
+    L = gpu.Loops(x, y)
+    yield x, y, L
+    L.close()
+
+
+def tuples(cxs, Ts, cy=0.0, geom=0):
+    tg, tcx, tcy, tT = [], [], [], []
+    for c in cxs:
+        for t in Ts:
+            tg.append(geom); tcx.append(c); tcy.append(cy); tT.append(t)
+    return np.array(tg), np.array(tcx), np.array(tcy), np.array(tT)
+
+
+def relerr(a, b):
+    scale = max(np.max(np.abs(b)), 1e-300)
+    return np.max(np.abs(a - b)) / scale
+
+
+def run_case(wfl, orc, loops, geoms, ogeoms, mode, nsub, spinor, prec, mapping, tup, splines=(), group=32):
+    x, y, L = loops
+    tg, tcx, tcy, tT = tup
+    b = wfl.Batch(list(geoms), tg, tcx, tcy, tT, mode=mode, precision=prec, nsub=nsub, spinor=spinor,
+                  group=group, splines=list(splines), want_perloop=True, mapping=mapping)
+    print(f"case mode={mode} nsub={nsub} spinor={spinor} prec={prec} mapping={mapping} ntup={len(tg)}", flush=True)
+    gs, pl = wfl.eval_batch(b, L)
+    rgs, rpl = orc.eval_batch(ogeoms, mode, nsub, spinor, x, y, group, tg, tcx, tcy, tT)
+    return gs, pl, rgs, rpl
+
+
+GAUSS_T = [0.01, 0.1, 0.7, 3.0]
+GAUSS_RHO = [0.0, 0.6, 1.7]
+
+
+@pytest.mark.parametrize("mapping", [1, 2])
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("spinor", [0, 1])
+def test_gaussian_fp64(gpu, orc, loops, mapping, mode, spinor):
+    geoms = [gpu.Geometry(kind=gpu.GAUSSIAN, F=1.0, l2=1.0)]
+    og = [orc.Geom(orc.GAUSSIAN, F=1.0, l2=1.0)]
+    gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, mode, 2, spinor, gpu.FP64, mapping,
+                                tuples(GAUSS_RHO, GAUSS_T))
+    e1, e2 = relerr(pl, rpl), relerr(gs, rgs)
+    print(f"  perloop {e1:.2e} gsum {e2:.2e}")
+    assert e1 < 1e-10 and e2 < 1e-10
+
+
+@pytest.mark.parametrize("mapping", [1, 2])
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("spinor", [0, 1])
+def test_gaussian_mixed(gpu, orc, loops, mapping, mode, spinor):
+    geoms = [gpu.Geometry(kind=gpu.GAUSSIAN, F=1.0, l2=1.0)]
+    og = [orc.Geom(orc.GAUSSIAN, F=1.0, l2=1.0)]
+    gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, mode, 2, spinor, gpu.MIXED, mapping,
+                                tuples(GAUSS_RHO, GAUSS_T))
+    e1, e2 = relerr(pl, rpl), relerr(gs, rgs)
+    print(f"  perloop {e1:.2e} gsum {e2:.2e}")
+    assert e1 < 2e-5 and e2 < 1e-5
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 2e-5)])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_nsub50_and_multi_geometry(gpu, orc, loops, prec, tol, mode):
+    """n = 50 (the reference's Simpson count) and several geometries per launch."""
+    specs = [(0.1, 0.25), (1.0, 1.0), (10.0, 4.0)]
+    geoms = [gpu.Geometry(kind=gpu.GAUSSIAN, F=F, l2=l2) for F, l2 in specs]
+    og = [orc.Geom(orc.GAUSSIAN, F=F, l2=l2) for F, l2 in specs]
+    parts = [tuples([0.0, 1.1], [0.05, 0.9], geom=g) for g in range(3)]
+    tup = tuple(np.concatenate([p[k] for p in parts]) for k in range(4))
+    for mapping in (1, 2):
+        gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, mode, 50, 1, prec, mapping, tup)
+        e1, e2 = relerr(pl, rpl), relerr(gs, rgs)
+        print(f"  mapping {mapping}: perloop {e1:.2e} gsum {e2:.2e}")
+        assert e1 < tol and e2 < tol
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 2e-5)])
+def test_lattice_images(gpu, orc, loops, prec, tol):
+    geoms = [gpu.Geometry(kind=gpu.GAUSSIAN, F=1.0, l2=0.25, lattice=1.5, images=1)]
+    og = [orc.Geom(orc.GAUSSIAN, F=1.0, l2=0.25, lattice=1.5, images=1)]
+    tg, tcx, tcy, tT = tuples([0.2, 0.75, 1.3], [0.05, 0.8, 4.0])
+    tcy = tcx * 0.5
+    for spinor in (0, 1):
+        gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, 1, 2, spinor, prec, 1, (tg, tcx, tcy, tT))
+        e1, e2 = relerr(pl, rpl), relerr(gs, rgs)
+        print(f"  spinor {spinor}: perloop {e1:.2e} gsum {e2:.2e}")
+        assert e1 < tol and e2 < tol
+
+
+def gaussian_spline(wfl, n=512, rho_max=20.0, l2=1.0):
+    rho = (np.arange(n) + 1) * (rho_max / n)        # periodic.c:129-151 knot layout
+    knots = (rho * rho).astype(np.float32)
+    fli = -np.expm1(-knots.astype(np.float64) / l2)
+    return knots, wfl.spline_build(fli, knots)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 5e-5)])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_spline_profile(gpu, orc, loops, prec, tol, mode):
+    knots, coefs = gaussian_spline(gpu)
+    geoms = [gpu.Geometry(kind=gpu.SPLINE, F=1.0, l2=1.0, a=1.0, spline_table=0)]
+    og = [orc.Geom(orc.SPLINE, F=1.0, l2=1.0, tubedist=1.0, knots=knots, coefs=coefs)]
+    for spinor in (0, 1):
+        gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, mode, 2, spinor, prec, 2,
+                                    tuples([0.0, 1.2], [0.02, 0.5, 3.0]), splines=[(knots, coefs)])
+        e1, e2 = relerr(pl, rpl), relerr(gs, rgs)
+        print(f"  spinor {spinor}: perloop {e1:.2e} gsum {e2:.2e}")
+        assert e1 < tol and e2 < tol
+
+
+@pytest.mark.parametrize("kind", ["STEP", "SMOOTH", "QUADRATIC", "PERIODIC", "FIXFLUX", "FIXFLUX_FF"])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_other_profiles_fp64(gpu, orc, loops, kind, mode):
+    k = getattr(gpu, kind)
+    geoms = [gpu.Geometry(kind=k, F=0.5, l2=0.64, a=2.0, lam_min=0.2)]
+    og = [orc.Geom(getattr(orc, kind), F=0.5, l2=0.64, tubedist=2.0, lmin=0.2)]
+    gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, mode, 2, 1, gpu.FP64, 2,
+                                tuples([0.0, 0.9], [0.03, 0.6]))
+    e1, e2 = relerr(pl, rpl), relerr(gs, rgs)
+    print(f"  {kind}: perloop {e1:.2e} gsum {e2:.2e}")
+    assert e1 < 1e-9 and e2 < 1e-9

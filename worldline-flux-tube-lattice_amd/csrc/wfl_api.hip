@@ -1,0 +1,599 @@
+// wfl_api.hip -- device side of the C ABI (include/wfl.h): loop ensembles,
+// batched plans, the EV() launch helper, and the small data-movement and
+// reduction kernels.  All functions exported here are extern "C".
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <algorithm>
+#include <vector>
+#include "wfl_internal.h"
+#include "wfl_kernels.h"
+
+using namespace wfl;
+
+extern "C" int wfl_launch_tuple_f32(const WflLaunch* a);
+extern "C" int wfl_launch_loop_f32(const WflLaunch* a);
+extern "C" int wfl_launch_tuple_f64(const WflLaunch* a);
+extern "C" int wfl_launch_loop_f64(const WflLaunch* a);
+
+// ---------------------------------------------------------------- errors
+static thread_local char g_err[512] = "";
+
+extern "C" void wfl_set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+extern "C" const char* wfl_last_error(void) { return g_err; }
+
+#define HIP_TRY(expr)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      wfl_set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return (e_ == hipErrorOutOfMemory) ? WFL_ERR_NOMEM : WFL_ERR_DEVICE;                  \
+    }                                                                                       \
+  } while (0)
+
+extern "C" int wfl_device_count(int* count) {
+  if (!count) { wfl_set_error("wfl_device_count: NULL"); return WFL_ERR_ARG; }
+  *count = 0;
+  hipError_t e = hipGetDeviceCount(count);
+  if (e != hipSuccess) { *count = 0; wfl_set_error("hipGetDeviceCount: %s", hipGetErrorString(e)); return WFL_ERR_DEVICE; }
+  return WFL_OK;
+}
+
+extern "C" int wfl_device_sync(void) {
+  HIP_TRY(hipDeviceSynchronize());
+  return WFL_OK;
+}
+
+extern "C" int wfl_internal_d2h(void* dst, const void* src, size_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return WFL_OK;
+}
+
+static int require_device() {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n < 1) {
+    wfl_set_error("no HIP device available (%s)", e != hipSuccess ? hipGetErrorString(e) : "count = 0");
+    return WFL_ERR_DEVICE;
+  }
+  return WFL_OK;
+}
+
+// ------------------------------------------------------ data kernels
+// Loop records (see wfl_kernels.h): w, dw = w_succ - w, e = w_succ - w_pred.
+// Differences of FP32 inputs are exact in double and rounded once to R.
+template <class R>
+__global__ void k_records(const float* __restrict__ x, const float* __restrict__ y, int Nl, int N,
+                          R* __restrict__ rec) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)Nl * N) return;
+  int l = (int)(idx / N), i = (int)(idx % N);
+  const float* xl = x + (size_t)l * N;
+  const float* yl = y + (size_t)l * N;
+  int s = succ_idx(i, N), p = pred_idx(i, N);
+  double wx = xl[i], wy = yl[i];
+  R* r = rec + (size_t)idx * 6;
+  r[0] = (R)wx;
+  r[1] = (R)wy;
+  r[2] = (R)((double)xl[s] - wx);
+  r[3] = (R)((double)yl[s] - wy);
+  r[4] = (R)((double)xl[s] - (double)xl[p]);
+  r[5] = (R)((double)yl[s] - (double)yl[p]);
+}
+
+__global__ void k_aos_to_soa(const float4* __restrict__ aos, long long n, float* __restrict__ x, float* __restrict__ y) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 v = aos[i];
+  x[i] = v.x;
+  y[i] = v.y;
+}
+
+__global__ void k_zero_u64(unsigned long long* p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0ull;
+}
+
+// Group sums of per-loop W - 1 (wave-per-loop mapping): one thread per
+// (tuple, group), sequential FP64 sum in loop order.
+__global__ void k_group_sums(const double* __restrict__ wm1, const DevTuple* __restrict__ tups, int ntup, int Nl,
+                             int group, int ngroups, double* __restrict__ gsum) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)ntup * ngroups) return;
+  int t = (int)(idx / ngroups), q = (int)(idx % ngroups);
+  int nl = tups[t].nloops;
+  int l0 = q * group, l1 = std::min(l0 + group, nl);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int l = l0; l < l1; ++l) {
+    const double* p = wm1 + ((size_t)t * Nl + l) * 3;
+    s0 += p[0]; s1 += p[1]; s2 += p[2];
+  }
+  double* o = gsum + idx * 3;
+  o[0] = s0; o[1] = s1; o[2] = s2;
+}
+
+// ------------------------------------------------------ loop ensembles
+struct wfl_loops_s {
+  int Nl = 0, N = 0;
+  float* x = nullptr;      // SoA [Nl*N]
+  float* y = nullptr;
+  float* rec32 = nullptr;  // [Nl][N][6]
+  double* rec64 = nullptr;
+};
+
+static int loops_alloc(wfl_loops** out, int Nl, int N) {
+  if (!out) { wfl_set_error("loops: NULL output"); return WFL_ERR_ARG; }
+  *out = nullptr;
+  if (Nl < 1 || N < 4 || N % 4 != 0) {
+    wfl_set_error("loops: need Nl >= 1 and N a positive multiple of 4 (got Nl=%d, N=%d)", Nl, N);
+    return WFL_ERR_ARG;
+  }
+  int rc = require_device();
+  if (rc) return rc;
+  wfl_loops* L = new wfl_loops();
+  L->Nl = Nl; L->N = N;
+  size_t bytes = sizeof(float) * (size_t)Nl * N;
+  hipError_t e1 = hipMalloc(&L->x, bytes);
+  hipError_t e2 = (e1 == hipSuccess) ? hipMalloc(&L->y, bytes) : e1;
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    if (L->x) (void)hipFree(L->x);
+    delete L;
+    wfl_set_error("loops: hipMalloc of %zu bytes failed", 2 * bytes);
+    return WFL_ERR_NOMEM;
+  }
+  *out = L;
+  return WFL_OK;
+}
+
+extern "C" void wfl_loops_destroy(wfl_loops* L) {
+  if (!L) return;
+  if (L->x) (void)hipFree(L->x);
+  if (L->y) (void)hipFree(L->y);
+  if (L->rec32) (void)hipFree(L->rec32);
+  if (L->rec64) (void)hipFree(L->rec64);
+  delete L;
+}
+
+extern "C" int wfl_loops_create(wfl_loops** out, const float* x, const float* y, int Nl, int N) {
+  if (!x || !y) { wfl_set_error("wfl_loops_create: NULL coordinates"); return WFL_ERR_ARG; }
+  int rc = loops_alloc(out, Nl, N);
+  if (rc) return rc;
+  size_t bytes = sizeof(float) * (size_t)Nl * N;
+  hipError_t e = hipMemcpy((*out)->x, x, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy((*out)->y, y, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    wfl_loops_destroy(*out); *out = nullptr;
+    wfl_set_error("wfl_loops_create: upload failed: %s", hipGetErrorString(e));
+    return WFL_ERR_DEVICE;
+  }
+  return WFL_OK;
+}
+
+extern "C" int wfl_loops_from_aos_device(wfl_loops** out, const void* aos, int Nl, int N) {
+  if (!aos) { wfl_set_error("wfl_loops_from_aos_device: NULL"); return WFL_ERR_ARG; }
+  int rc = loops_alloc(out, Nl, N);
+  if (rc) return rc;
+  long long n = (long long)Nl * N;
+  hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0,
+                     (const float4*)aos, n, (*out)->x, (*out)->y);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    wfl_loops_destroy(*out); *out = nullptr;
+    wfl_set_error("AoS->SoA conversion failed: %s", hipGetErrorString(e));
+    return WFL_ERR_DEVICE;
+  }
+  return WFL_OK;
+}
+
+extern "C" int wfl_gen_loops(uint32_t config_id, int Nl, int N, float* x, float* y);
+
+extern "C" int wfl_loops_generate(wfl_loops** out, uint32_t config_id, int Nl, int N) {
+  if (Nl < 1 || N < 4 || N % 4 != 0) { wfl_set_error("wfl_loops_generate: bad shape"); return WFL_ERR_ARG; }
+  size_t n = (size_t)Nl * N;
+  float* x = (float*)malloc(n * sizeof(float));
+  float* y = (float*)malloc(n * sizeof(float));
+  if (!x || !y) { free(x); free(y); wfl_set_error("wfl_loops_generate: host malloc"); return WFL_ERR_NOMEM; }
+  int rc = wfl_gen_loops(config_id, Nl, N, x, y);
+  if (rc == WFL_OK) rc = wfl_loops_create(out, x, y, Nl, N);
+  free(x); free(y);
+  return rc;
+}
+
+extern "C" int wfl_loops_shape(const wfl_loops* L, int* Nl, int* N) {
+  if (!L) { wfl_set_error("wfl_loops_shape: NULL"); return WFL_ERR_ARG; }
+  if (Nl) *Nl = L->Nl;
+  if (N) *N = L->N;
+  return WFL_OK;
+}
+
+static int loops_records(wfl_loops* L, int precision) {
+  long long n = (long long)L->Nl * L->N;
+  dim3 grid((unsigned)((n + 255) / 256)), block(256);
+  if (precision == WFL_MIXED && !L->rec32) {
+    HIP_TRY(hipMalloc(&L->rec32, sizeof(float) * 6 * (size_t)n));
+    hipLaunchKernelGGL(k_records<float>, grid, block, 0, 0, L->x, L->y, L->Nl, L->N, L->rec32);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+  }
+  if (precision == WFL_FP64 && !L->rec64) {
+    HIP_TRY(hipMalloc(&L->rec64, sizeof(double) * 6 * (size_t)n));
+    hipLaunchKernelGGL(k_records<double>, grid, block, 0, 0, L->x, L->y, L->Nl, L->N, L->rec64);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+  }
+  return WFL_OK;
+}
+
+// ------------------------------------------------------ geometry upload
+static void fill_geom(const wfl_geometry& g, int spline_off, int spline_n, DevGeom& d) {
+  memset(&d, 0, sizeof d);
+  d.kind = g.kind;
+  d.images = g.images;
+  d.spline_off = spline_off;
+  d.spline_n = spline_n;
+  d.Ff = (float)g.F;       d.F = d.Ff;
+  d.l2f = (float)g.l2;     d.l2 = d.l2f;
+  d.inv_l2 = 1.0 / d.l2;   d.inv_l2f = (float)d.inv_l2;
+  d.af = (float)g.a;       d.a = d.af;
+  d.lminf = (float)g.lam_min; d.lmin = d.lminf;
+  d.q1f = (float)g.fix_q1; d.q1 = d.q1f;
+  d.q2f = (float)g.fix_q2; d.q2 = d.q2f;
+  d.latticef = (float)g.lattice; d.lattice = g.lattice > 0.0 ? (double)d.latticef : 0.0;
+  double k = -1.4426950408889634074 / d.l2;
+  d.k_hi = (float)k;
+  d.k_lo = (float)(k - (double)d.k_hi);
+}
+
+static int check_geom(const wfl_geometry& g, int nspline_tables, const wfl_spline_table* tabs) {
+  if (g.kind < WFL_STEP || g.kind > WFL_FIXFLUX_FF) { wfl_set_error("geometry: unknown profile %d", g.kind); return WFL_ERR_ARG; }
+  if (!(g.l2 > 0.0) || !isfinite(g.F)) { wfl_set_error("geometry: need l2 > 0 and finite F"); return WFL_ERR_ARG; }
+  if (g.lattice > 0.0 && (g.images < 0 || g.images > 16)) { wfl_set_error("geometry: images must be in [0, 16]"); return WFL_ERR_ARG; }
+  if ((g.kind == WFL_PERIODIC || g.kind == WFL_FIXFLUX || g.kind == WFL_FIXFLUX_FF) && !(g.a > 0.0)) {
+    wfl_set_error("geometry: profile %d needs tubedist a > 0", g.kind); return WFL_ERR_ARG;
+  }
+  if (g.kind == WFL_SPLINE) {
+    if (g.spline_table < 0 || g.spline_table >= nspline_tables || !tabs) { wfl_set_error("geometry: bad spline table index %d", g.spline_table); return WFL_ERR_ARG; }
+    const wfl_spline_table& t = tabs[g.spline_table];
+    if (t.n < 2 || !t.rho2 || !t.coefs) { wfl_set_error("spline table %d: need n >= 2 and data", g.spline_table); return WFL_ERR_ARG; }
+    for (int i = 1; i < t.n; ++i)
+      if (!(t.rho2[i] > t.rho2[i - 1])) { wfl_set_error("spline table %d: knots not increasing at %d", g.spline_table, i); return WFL_ERR_ARG; }
+  }
+  return WFL_OK;
+}
+
+// ------------------------------------------------------ plans
+struct wfl_plan_s {
+  wfl_batch b{};
+  wfl_loops* L = nullptr;
+  int ngroups = 0, mapping = 1, prof_gauss = 1, ntasks = 0;
+  std::vector<DevTuple> tups;
+  DevTuple* d_tups = nullptr;
+  DevGeom* d_geoms = nullptr;
+  int* d_lanes = nullptr;
+  Task* d_tasks = nullptr;
+  float* d_knots = nullptr;
+  float4* d_coefs = nullptr;
+  double* d_gsum = nullptr;
+  double* d_perloop = nullptr;
+  double* d_wm1 = nullptr;
+  unsigned long long* d_nonfinite = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipStream_t gstream = nullptr;
+};
+
+extern "C" void wfl_plan_destroy(wfl_plan* p) {
+  if (!p) return;
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->gstream) (void)hipStreamDestroy(p->gstream);
+  void* bufs[] = {p->d_tups, p->d_geoms, p->d_lanes, p->d_tasks, p->d_knots, p->d_coefs,
+                  p->d_gsum, p->d_perloop, p->d_wm1, p->d_nonfinite};
+  for (void* b : bufs) if (b) (void)hipFree(b);
+  delete p;
+}
+
+template <class T>
+static int upload(T** dst, const T* src, size_t n) {
+  if (n == 0) n = 1;
+  HIP_TRY(hipMalloc((void**)dst, sizeof(T) * n));
+  if (src) HIP_TRY(hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice));
+  return WFL_OK;
+}
+
+#define TRY(expr) do { int rc_ = (expr); if (rc_) { wfl_plan_destroy(p); return rc_; } } while (0)
+
+extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L) {
+  if (!out || !b || !L) { wfl_set_error("wfl_plan_create: NULL argument"); return WFL_ERR_ARG; }
+  *out = nullptr;
+  if (b->mode != WFL_MODE_REFERENCE && b->mode != WFL_MODE_PHYSICAL) { wfl_set_error("bad mode %d", b->mode); return WFL_ERR_ARG; }
+  if (b->precision != WFL_FP64 && b->precision != WFL_MIXED) { wfl_set_error("bad precision %d", b->precision); return WFL_ERR_ARG; }
+  if (b->nsub < 2 || b->nsub % 2 != 0 || b->nsub > 4096) { wfl_set_error("nsub must be even and in [2, 4096] (got %d)", b->nsub); return WFL_ERR_ARG; }
+  if (b->group < 1) { wfl_set_error("group must be >= 1"); return WFL_ERR_ARG; }
+  if (b->ngeom < 1 || !b->geoms) { wfl_set_error("need at least one geometry"); return WFL_ERR_ARG; }
+  if (b->ntup < 0 || (b->ntup > 0 && (!b->tup_geom || !b->tup_cx || !b->tup_cy || !b->tup_T))) {
+    wfl_set_error("tuple arrays missing"); return WFL_ERR_ARG;
+  }
+  if (b->mapping < 0 || b->mapping > 2) { wfl_set_error("bad mapping %d", b->mapping); return WFL_ERR_ARG; }
+  for (int g = 0; g < b->ngeom; ++g) { int rc = check_geom(b->geoms[g], b->nspline, b->splines); if (rc) return rc; }
+  for (int t = 0; t < b->ntup; ++t) {
+    if (b->tup_geom[t] < 0 || b->tup_geom[t] >= b->ngeom) { wfl_set_error("tuple %d: geometry index out of range", t); return WFL_ERR_ARG; }
+    if (!(b->tup_T[t] > 0.0) || !isfinite(b->tup_T[t]) || !isfinite(b->tup_cx[t]) || !isfinite(b->tup_cy[t])) {
+      wfl_set_error("tuple %d: need finite x_cm and T > 0", t); return WFL_ERR_ARG;
+    }
+  }
+  int rc0 = require_device();
+  if (rc0) return rc0;
+
+  wfl_plan* p = new wfl_plan();
+  p->b = *b;
+  p->b.geoms = nullptr; p->b.splines = nullptr; p->b.tup_geom = nullptr;
+  p->b.tup_cx = p->b.tup_cy = p->b.tup_T = nullptr; p->b.geom_nloops = nullptr;
+  p->L = L;
+  const int Nl = L->Nl, N = L->N;
+  p->ngroups = (Nl + b->group - 1) / b->group;
+
+  // splines: concatenated knots / coefficients
+  std::vector<int> sp_off(b->nspline > 0 ? b->nspline : 1, 0);
+  std::vector<float> knots;
+  std::vector<float4> coefs;
+  for (int s = 0; s < b->nspline; ++s) {
+    sp_off[s] = (int)knots.size();
+    const wfl_spline_table& t = b->splines[s];
+    for (int i = 0; i < t.n; ++i) {
+      knots.push_back(t.rho2[i]);
+      coefs.push_back(make_float4(t.coefs[4 * i], t.coefs[4 * i + 1], t.coefs[4 * i + 2], t.coefs[4 * i + 3]));
+    }
+  }
+  std::vector<DevGeom> geoms(b->ngeom);
+  p->prof_gauss = 1;
+  for (int g = 0; g < b->ngeom; ++g) {
+    const wfl_geometry& G = b->geoms[g];
+    int off = 0, n = 0;
+    if (G.kind == WFL_SPLINE) { off = sp_off[G.spline_table]; n = b->splines[G.spline_table].n; }
+    fill_geom(G, off, n, geoms[g]);
+    if (G.kind != WFL_GAUSSIAN) p->prof_gauss = 0;
+  }
+  // tuples (FP32-quantized parameters, image centre in double)
+  p->tups.resize(b->ntup);
+  for (int t = 0; t < b->ntup; ++t) {
+    DevTuple& d = p->tups[t];
+    const DevGeom& G = geoms[b->tup_geom[t]];
+    d.geom = b->tup_geom[t];
+    d.cx = (float)b->tup_cx[t];
+    d.cy = (float)b->tup_cy[t];
+    d.rtT = (float)sqrt(b->tup_T[t]);
+    d.pad = 0.0f;
+    d.m0 = d.n0 = 0;
+    if (G.lattice > 0.0) {
+      d.m0 = (int)floor((double)d.cx / G.lattice + 0.5);
+      d.n0 = (int)floor((double)d.cy / G.lattice + 0.5);
+    }
+    int nl = Nl;
+    if (b->geom_nloops) nl = std::max(0, std::min(Nl, b->geom_nloops[d.geom]));
+    d.nloops = nl;
+  }
+  // lane groups: tuples sorted by geometry, each geometry padded to 64 lanes
+  std::vector<int> order(b->ntup);
+  for (int t = 0; t < b->ntup; ++t) order[t] = t;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return p->tups[a].geom < p->tups[c].geom; });
+  std::vector<int> lanes;
+  std::vector<int> lg_nloops;
+  for (size_t k = 0; k < order.size();) {
+    int g = p->tups[order[k]].geom;
+    size_t e = k;
+    while (e < order.size() && p->tups[order[e]].geom == g) ++e;
+    for (size_t s = k; s < e; s += 64) {
+      int nl = 0;
+      for (size_t q = 0; q < 64; ++q) {
+        int v = (s + q < e) ? order[s + q] : -1;
+        lanes.push_back(v);
+        if (v >= 0) nl = std::max(nl, p->tups[v].nloops);
+      }
+      lg_nloops.push_back(nl);
+    }
+    k = e;
+  }
+  int nlg = (int)lg_nloops.size();
+  std::vector<Task> tasks;
+  for (int q = 0; q < p->ngroups; ++q)
+    for (int lg = 0; lg < nlg; ++lg) {
+      int l0 = q * b->group, l1 = std::min(l0 + b->group, lg_nloops[lg]);
+      if (l1 > l0) tasks.push_back(Task{lg, l0, l1 - l0, q});
+    }
+  p->ntasks = (int)tasks.size();
+  // mapping (autotune heuristic: lane-per-tuple needs enough lane groups)
+  int mapping = b->mapping;
+  if (mapping == 0) {
+    double util = b->ntup > 0 ? (double)b->ntup / (64.0 * std::max(nlg, 1)) : 0.0;
+    mapping = (util >= 0.5 && p->ntasks >= 512) ? 1 : 2;
+  }
+  p->mapping = mapping;
+
+  TRY(loops_records(L, b->precision));
+  TRY(upload(&p->d_tups, p->tups.data(), p->tups.size()));
+  TRY(upload(&p->d_geoms, geoms.data(), geoms.size()));
+  TRY(upload(&p->d_lanes, lanes.data(), lanes.size()));
+  TRY(upload(&p->d_tasks, tasks.data(), tasks.size()));
+  TRY(upload(&p->d_knots, knots.data(), knots.size()));
+  TRY(upload(&p->d_coefs, coefs.data(), coefs.size()));
+  size_t ng3 = (size_t)std::max(b->ntup, 1) * p->ngroups * 3;
+  TRY(upload<double>(&p->d_gsum, nullptr, ng3));
+  {
+    hipError_t e = hipMemset(p->d_gsum, 0, sizeof(double) * ng3);
+    if (e != hipSuccess) { wfl_set_error("hipMemset: %s", hipGetErrorString(e)); wfl_plan_destroy(p); return WFL_ERR_DEVICE; }
+  }
+  if (b->want_perloop) {
+    size_t n6 = (size_t)std::max(b->ntup, 1) * Nl * 6;
+    TRY(upload<double>(&p->d_perloop, nullptr, n6));
+    hipError_t e = hipMemset(p->d_perloop, 0, sizeof(double) * n6);
+    if (e != hipSuccess) { wfl_set_error("hipMemset: %s", hipGetErrorString(e)); wfl_plan_destroy(p); return WFL_ERR_DEVICE; }
+  }
+  if (mapping == 2) TRY(upload<double>(&p->d_wm1, nullptr, (size_t)std::max(b->ntup, 1) * Nl * 3));
+  TRY(upload<unsigned long long>(&p->d_nonfinite, nullptr, 1));
+  {
+    hipError_t e = hipMemset(p->d_nonfinite, 0, sizeof(unsigned long long));
+    if (e != hipSuccess) { wfl_set_error("hipMemset: %s", hipGetErrorString(e)); wfl_plan_destroy(p); return WFL_ERR_DEVICE; }
+  }
+  (void)N;
+  *out = p;
+  return WFL_OK;
+}
+
+static void fill_launch(const wfl_plan* p, WflLaunch& a, hipStream_t s) {
+  memset(&a, 0, sizeof a);
+  a.rec = p->b.precision == WFL_MIXED ? (const void*)p->L->rec32 : (const void*)p->L->rec64;
+  a.N = p->L->N; a.Nl = p->L->Nl;
+  a.tups = p->d_tups; a.geoms = p->d_geoms; a.ntup = p->b.ntup;
+  a.lanes = p->d_lanes; a.tasks = p->d_tasks; a.ntasks = p->ntasks; a.ngroups = p->ngroups;
+  a.knots = p->d_knots; a.coefs = p->d_coefs;
+  a.prof_gauss = p->prof_gauss; a.spinor = p->b.spinor ? 1 : 0; a.mode = p->b.mode; a.nsub = p->b.nsub;
+  a.gsum = p->d_gsum; a.perloop = p->d_perloop; a.wm1 = p->d_wm1;
+  a.nonfinite = p->d_nonfinite;
+  a.stream = s;
+}
+
+extern "C" int wfl_plan_run(wfl_plan* p, void* stream) {
+  if (!p) { wfl_set_error("wfl_plan_run: NULL plan"); return WFL_ERR_STATE; }
+  if (p->b.ntup == 0) return WFL_OK;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, p->d_nonfinite);
+  HIP_TRY(hipGetLastError());
+  WflLaunch a;
+  fill_launch(p, a, s);
+  int e;
+  if (p->mapping == 1) {
+    e = p->b.precision == WFL_MIXED ? wfl_launch_tuple_f32(&a) : wfl_launch_tuple_f64(&a);
+  } else {
+    e = p->b.precision == WFL_MIXED ? wfl_launch_loop_f32(&a) : wfl_launch_loop_f64(&a);
+    if (e == 0) {
+      long long n = (long long)p->b.ntup * p->ngroups;
+      hipLaunchKernelGGL(k_group_sums, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, p->d_wm1, p->d_tups,
+                         p->b.ntup, p->L->Nl, p->b.group, p->ngroups, p->d_gsum);
+      e = (int)hipGetLastError();
+    }
+  }
+  if (e) { wfl_set_error("kernel launch failed: %s", hipGetErrorString((hipError_t)e)); return WFL_ERR_DEVICE; }
+  return WFL_OK;
+}
+
+extern "C" int wfl_plan_graph_run(wfl_plan* p, void* stream) {
+  if (!p) { wfl_set_error("wfl_plan_graph_run: NULL plan"); return WFL_ERR_STATE; }
+  if (!p->gexec) {
+    if (!p->gstream) HIP_TRY(hipStreamCreateWithFlags(&p->gstream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamBeginCapture(p->gstream, hipStreamCaptureModeThreadLocal));
+    int rc = wfl_plan_run(p, p->gstream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(p->gstream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) { wfl_set_error("graph capture failed: %s", hipGetErrorString(e)); return WFL_ERR_DEVICE; }
+    p->graph = g;
+    HIP_TRY(hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0));
+  }
+  HIP_TRY(hipGraphLaunch(p->gexec, (hipStream_t)stream));
+  return WFL_OK;
+}
+
+extern "C" int wfl_plan_read(wfl_plan* p, double* gsum, double* perloop) {
+  if (!p) { wfl_set_error("wfl_plan_read: NULL plan"); return WFL_ERR_STATE; }
+  HIP_TRY(hipDeviceSynchronize());
+  if (gsum && p->b.ntup > 0)
+    HIP_TRY(hipMemcpy(gsum, p->d_gsum, sizeof(double) * (size_t)p->b.ntup * p->ngroups * 3, hipMemcpyDeviceToHost));
+  if (perloop) {
+    if (!p->d_perloop) { wfl_set_error("wfl_plan_read: per-loop outputs were not requested"); return WFL_ERR_STATE; }
+    HIP_TRY(hipMemcpy(perloop, p->d_perloop, sizeof(double) * (size_t)p->b.ntup * p->L->Nl * 6, hipMemcpyDeviceToHost));
+  }
+  return WFL_OK;
+}
+
+extern "C" int wfl_plan_ngroups(const wfl_plan* p) { return p ? p->ngroups : 0; }
+extern "C" const double* wfl_plan_gsum_device(const wfl_plan* p) { return p ? p->d_gsum : nullptr; }
+extern "C" int wfl_plan_mapping(const wfl_plan* p) { return p ? p->mapping : 0; }
+
+extern "C" int wfl_plan_nonfinite(wfl_plan* p, long long* count) {
+  if (!p || !count) { wfl_set_error("wfl_plan_nonfinite: NULL"); return WFL_ERR_ARG; }
+  unsigned long long v = 0;
+  HIP_TRY(hipMemcpy(&v, p->d_nonfinite, sizeof v, hipMemcpyDeviceToHost));
+  *count = (long long)v;
+  return WFL_OK;
+}
+
+extern "C" int wfl_eval_batch(const wfl_batch* b, wfl_loops* L, double* gsum, double* perloop) {
+  wfl_plan* p = nullptr;
+  int rc = wfl_plan_create(&p, b, L);
+  if (rc) return rc;
+  rc = wfl_plan_run(p, nullptr);
+  if (!rc) rc = wfl_plan_read(p, gsum, perloop);
+  wfl_plan_destroy(p);
+  return rc;
+}
+
+// ------------------------------------------------------ EV() helper
+// Loop copies of the caller's AoS device buffer, reused while the pointer
+// and shape stay the same.
+static struct {
+  const void* aos = nullptr;
+  int Nl = 0, N = 0;
+  wfl_loops* L = nullptr;
+  DevTuple* d_tup = nullptr;
+  DevGeom* d_geom = nullptr;
+  unsigned long long* d_nf = nullptr;
+} g_ev;
+
+extern "C" void wfl_internal_ev_release(void) {
+  if (g_ev.L) wfl_loops_destroy(g_ev.L);
+  if (g_ev.d_tup) (void)hipFree(g_ev.d_tup);
+  if (g_ev.d_geom) (void)hipFree(g_ev.d_geom);
+  if (g_ev.d_nf) (void)hipFree(g_ev.d_nf);
+  g_ev.L = nullptr; g_ev.d_tup = nullptr; g_ev.d_geom = nullptr; g_ev.d_nf = nullptr;
+  g_ev.aos = nullptr; g_ev.Nl = g_ev.N = 0;
+}
+
+extern "C" int wfl_internal_ev_launch(const void* dev_aos, int Nl, int N, const wfl_geometry* g,
+                                      const float* knots_d, const float* coefs_d, int nspline,
+                                      int mode, int precision, int nsub, int fermion,
+                                      float cx, float cy, float rtT, void* ws_scal_d, void* ws_ferm_d) {
+  if (!dev_aos || !g || !ws_scal_d || (fermion == 1 && !ws_ferm_d)) { wfl_set_error("EV: NULL device buffer in Wparams"); return WFL_ERR_ARG; }
+  if (Nl < 1 || N < 4 || N % 4 != 0) { wfl_set_error("EV: need Nl >= 1 and Nppl a multiple of 4 (got %d, %d)", Nl, N); return WFL_ERR_ARG; }
+  if (g->kind == WFL_SPLINE && (!knots_d || !coefs_d || nspline < 2)) { wfl_set_error("EV: spline profile needs rho2sp and flcoefs"); return WFL_ERR_ARG; }
+  if (nsub < 2 || nsub % 2) { wfl_set_error("EV: nsub must be even"); return WFL_ERR_ARG; }
+  if (!(g_ev.aos == dev_aos && g_ev.Nl == Nl && g_ev.N == N && g_ev.L)) {
+    wfl_internal_ev_release();
+    int rc = wfl_loops_from_aos_device(&g_ev.L, dev_aos, Nl, N);
+    if (rc) return rc;
+    g_ev.aos = dev_aos; g_ev.Nl = Nl; g_ev.N = N;
+    HIP_TRY(hipMalloc(&g_ev.d_tup, sizeof(DevTuple)));
+    HIP_TRY(hipMalloc(&g_ev.d_geom, sizeof(DevGeom)));
+    HIP_TRY(hipMalloc(&g_ev.d_nf, sizeof(unsigned long long)));
+  }
+  int rc = loops_records(g_ev.L, precision);
+  if (rc) return rc;
+  DevGeom dg;
+  fill_geom(*g, 0, g->kind == WFL_SPLINE ? nspline : 0, dg);
+  dg.lattice = 0.0; dg.latticef = 0.0f; dg.images = 0;
+  DevTuple dt;
+  memset(&dt, 0, sizeof dt);
+  dt.geom = 0; dt.cx = cx; dt.cy = cy; dt.rtT = rtT; dt.nloops = Nl;
+  HIP_TRY(hipMemcpy(g_ev.d_geom, &dg, sizeof dg, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(g_ev.d_tup, &dt, sizeof dt, hipMemcpyHostToDevice));
+  WflLaunch a;
+  memset(&a, 0, sizeof a);
+  a.rec = precision == WFL_MIXED ? (const void*)g_ev.L->rec32 : (const void*)g_ev.L->rec64;
+  a.N = N; a.Nl = Nl; a.tups = g_ev.d_tup; a.geoms = g_ev.d_geom; a.ntup = 1;
+  a.knots = knots_d; a.coefs = (const float4*)coefs_d;
+  a.prof_gauss = g->kind == WFL_GAUSSIAN; a.spinor = fermion == 1; a.mode = mode; a.nsub = nsub;
+  a.ws_scal = (float4*)ws_scal_d; a.ws_ferm = fermion == 1 ? (float4*)ws_ferm_d : nullptr;
+  a.nonfinite = g_ev.d_nf;
+  a.stream = 0;
+  int e = precision == WFL_MIXED ? wfl_launch_loop_f32(&a) : wfl_launch_loop_f64(&a);
+  if (e) { wfl_set_error("EV kernel launch failed: %s", hipGetErrorString((hipError_t)e)); return WFL_ERR_DEVICE; }
+  return WFL_OK;
+}

@@ -1,0 +1,534 @@
+/*
+ * wfl_host.c -- host side of libwfl.so, in C like the reference's host code.
+ *
+ *   EV          drop-in for intEV.cu:111-148 (launch via the HIP C ABI, D2H,
+ *               EVMean); repeated calls at the same (x_cm, T, F, l2) reuse
+ *               the device results instead of relaunching (the reference
+ *               relaunches once per loop group, Integrand.c:249-301).
+ *   EVMean      Integrand.c:325-407 (ensemble average, Richardson)
+ *   Tfunc       Integrand.c:195-316 (proper-time integrand policy)
+ *   integrateT  Tint.c:15-33, 100-175 (Simpson in t, T = (1-t)/t)
+ *   Exact, MagField, smTscal, smTferm   Integrand.c:20-172
+ *   getwl       getwl.c:6-40 (text loops into float4 AoS)
+ *   wfl_gen_loops     seeded unit loops (Philox4x32-10 + Box-Muller)
+ *   wfl_spline_build  periodic.c:45-110 (clamped cubic spline, zero slopes)
+ *   wfl_logsimpson, wfl_leff_groups   T-integral on log-spaced grids
+ */
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "../../include/intT.h"
+#include "wfl_internal.h"
+
+static const double WFL_E = 0.30282212;   /* Integrand.c:14 */
+static const double WFL_M2 = 1.0;         /* Integrand.c:12 */
+static const double WFL_PI = 3.14159265358979323846;
+
+/* ------------------------------------------------------------------ */
+/* Drop-in configuration (the reference fixes these in intT.h)          */
+/* ------------------------------------------------------------------ */
+static wfl_geometry g_prof = {WFL_GAUSSIAN, 1.0, 1.0, 1.0, 0.1, 0.443993816168079, 0.0742477533879612, 0.0, 0, -1};
+static int g_nspline = 512;
+static int g_mode = WFL_MODE_REFERENCE, g_prec = WFL_FP64, g_nsub = 50;
+static long long g_ev_launches = 0;
+static unsigned g_cfg_version = 1;
+
+static struct {
+  int valid;
+  const void *aos, *sd, *fd, *kn, *cf;
+  int Nl, N, fermion;
+  float cx, cy, rtT, F, l2;
+  unsigned version;
+} g_evkey;
+
+int wfl_set_profile(const wfl_geometry *g, int nspline) {
+  if (!g) { wfl_set_error("wfl_set_profile: NULL"); return WFL_ERR_ARG; }
+  if (g->kind < WFL_STEP || g->kind > WFL_FIXFLUX_FF) { wfl_set_error("wfl_set_profile: unknown profile %d", g->kind); return WFL_ERR_ARG; }
+  if (g->kind == WFL_SPLINE && nspline < 2) { wfl_set_error("wfl_set_profile: spline needs nspline >= 2"); return WFL_ERR_ARG; }
+  g_prof = *g;
+  g_nspline = nspline;
+  g_cfg_version++;
+  return WFL_OK;
+}
+
+int wfl_set_numerics(int mode, int precision, int nsub) {
+  if ((mode != WFL_MODE_REFERENCE && mode != WFL_MODE_PHYSICAL) || (precision != WFL_FP64 && precision != WFL_MIXED) ||
+      nsub < 2 || nsub % 2) {
+    wfl_set_error("wfl_set_numerics: bad mode %d / precision %d / nsub %d", mode, precision, nsub);
+    return WFL_ERR_ARG;
+  }
+  g_mode = mode; g_prec = precision; g_nsub = nsub;
+  g_cfg_version++;
+  return WFL_OK;
+}
+
+void wfl_ev_reset(void) {
+  g_evkey.valid = 0;
+  g_ev_launches = 0;
+  wfl_internal_ev_release();
+}
+
+long long wfl_ev_launches(void) { return g_ev_launches; }
+
+/* ------------------------------------------------------------------ */
+/* EV and EVMean                                                        */
+/* ------------------------------------------------------------------ */
+double EV(double T, void *p, int *WLlist) {
+  struct Wparams params = *(struct Wparams *)p;
+  const int groupsize = 128;            /* intEV.cu:117 */
+  double ev = 0.0;
+  float rtT = (float)sqrt(T);
+  wfl_geometry g = g_prof;
+  g.F = params.F;
+  g.l2 = params.l2;
+  int fermion = params.fermion == 1;
+  int hit = g_evkey.valid && g_evkey.aos == (const void *)params.worldlines && g_evkey.sd == (const void *)params.Wsscal_d &&
+            g_evkey.fd == (const void *)params.Wsferm_d && g_evkey.kn == (const void *)params.rho2sp &&
+            g_evkey.cf == (const void *)params.flcoefs && g_evkey.Nl == params.Nl && g_evkey.N == params.Nppl &&
+            g_evkey.fermion == fermion && g_evkey.cx == params.xcm.x && g_evkey.cy == params.xcm.y &&
+            g_evkey.rtT == rtT && g_evkey.F == (float)params.F && g_evkey.l2 == (float)params.l2 &&
+            g_evkey.version == g_cfg_version;
+  if (!hit) {
+    g_evkey.valid = 0;
+    int rc = wfl_internal_ev_launch(params.worldlines, params.Nl, params.Nppl, &g, params.rho2sp,
+                                    (const float *)params.flcoefs, g_nspline, g_mode, g_prec, g_nsub, fermion,
+                                    params.xcm.x, params.xcm.y, rtT, params.Wsscal_d, params.Wsferm_d);
+    if (rc) {
+      printf("EV(): %s\n", wfl_last_error());   /* reported like intEV.cu:126-127 */
+      return NAN;
+    }
+    g_ev_launches++;
+    g_evkey.valid = 1;
+    g_evkey.aos = params.worldlines; g_evkey.sd = params.Wsscal_d; g_evkey.fd = params.Wsferm_d;
+    g_evkey.kn = params.rho2sp; g_evkey.cf = params.flcoefs;
+    g_evkey.Nl = params.Nl; g_evkey.N = params.Nppl; g_evkey.fermion = fermion;
+    g_evkey.cx = params.xcm.x; g_evkey.cy = params.xcm.y; g_evkey.rtT = rtT;
+    g_evkey.F = (float)params.F; g_evkey.l2 = (float)params.l2; g_evkey.version = g_cfg_version;
+  }
+  if (wfl_internal_d2h(params.Wsscal_h, params.Wsscal_d, (size_t)params.Nl * sizeof(float4))) {
+    printf("EV(): %s\n", wfl_last_error());
+    g_evkey.valid = 0;
+    return NAN;
+  }
+  if (fermion && wfl_internal_d2h(params.Wsferm_h, params.Wsferm_d, (size_t)params.Nl * sizeof(float4))) {
+    printf("EV(): %s\n", wfl_last_error());
+    g_evkey.valid = 0;
+    return NAN;
+  }
+  EVMean(&ev, params.Wsscal_h, params.Wsferm_h, groupsize, WLlist, T, params.fermion);
+  return ev;
+}
+
+/* abs() of a double in Integrand.c:371 converts to int first; on x86-64 an
+ * out-of-range conversion yields INT_MIN and the guard stays off. */
+static int guard600(double v) {
+  if (!(fabs(v) < 2147483648.0)) return 0;
+  return fabs(trunc(v)) > 600.0;
+}
+
+void EVMean(double *EVout, float4 *Wsscal_h, float4 *Wsferm_h, int n, int *WL, double T, int fermion) {
+  double e1 = 0.0, e2 = 0.0, e4 = 0.0;
+  for (int i = 0; i < n; ++i) {
+    int w = WL[i];
+    float4 s = Wsscal_h[w];
+    double nb4 = (double)(s.y + s.z);
+    double p1 = cos(0.5 * (double)s.x + 0.25 * nb4);
+    double p2 = cos((double)s.x);
+    double p4 = cos(0.5 * nb4);
+    if (fermion == 1) {
+      float4 f = Wsferm_h[w];
+      if (!isfinite(f.x) || !isfinite(f.y) || !isfinite(f.z)) printf("Warning: WSferm is infinite. Worldline=%d\n", w);
+      double fb4 = (double)(f.y + f.z);
+      if (guard600(fb4) || guard600((double)f.x)) {
+        printf("Warning: Large cosh argument.  T=%f, WLi=%d\n", T, w);
+        p1 *= 1.0e10; p2 *= 1.0e10; p4 *= 1.0e10;
+      } else {
+        p1 *= cosh(0.5 * (double)f.x + 0.25 * fb4);
+        p2 *= cosh((double)f.x);
+        p4 *= cosh(0.5 * fb4);
+      }
+      if (!isfinite(p1) || !isfinite(p2) || !isfinite(p4)) printf("Warning: Infinity detected: WL# %d\n", w);
+    }
+    e4 += p4; e2 += p2; e1 += p1;
+  }
+  e4 /= n; e2 /= n; e1 /= n;
+  *EVout = 8.0 / 3.0 * e1 - 2.0 * e2 + 1.0 / 3.0 * e4;
+}
+
+double wfl_group_ev_minus1(const double g3[3], int count) {
+  if (count <= 0) return 0.0;
+  return (8.0 / 3.0 * g3[0] - 2.0 * g3[1] + 1.0 / 3.0 * g3[2]) / (double)count;
+}
+
+/* ------------------------------------------------------------------ */
+/* Closed forms and the proper-time integrand policy                    */
+/* ------------------------------------------------------------------ */
+double Exact(double T, double B, int fermion) {
+  double TB = WFL_E * T * B;
+  if (B < 1.0e-40) return 0.0;
+  if (fermion == 1) return exp(-WFL_M2 * T) / (T * T * T) * (TB / tanh(TB) - 1.0 - TB * TB / 3.0);
+  if (TB < 50.0) return exp(-WFL_M2 * T) / (T * T * T) * (TB / sinh(TB) - 1.0 + TB * TB / 6.0);
+  return exp(-WFL_M2 * T) / (T * T * T) * (TB * TB / 6.0 - 1.0);
+}
+
+double smTscal(double B, double T) {
+  double B2 = WFL_E * WFL_E * B * B, B4 = B2 * B2, B6 = B2 * B4, m2 = WFL_M2;
+  double T3 = T * T * T, T4 = T3 * T;
+  return 7.0 / 360.0 * B4 * T * (1.0 - m2 * T) + (147.0 * B4 * m2 * m2 - 31.0 * B6) / 15120.0 * T3 +
+         (31.0 * B6 * m2 - 49.0 * B4 * m2 * m2 * m2) / 15120.0 * T4;
+}
+
+double smTferm(double B, double T) {
+  double B2 = WFL_E * WFL_E * B * B, B4 = B2 * B2, B6 = B2 * B4, m2 = WFL_M2;
+  double T3 = T * T * T, T4 = T3 * T;
+  return 1.0 / 45.0 * B4 * T * (m2 * T - 1.0) + (2.0 * B6 / 945.0 - B4 * m2 * m2 / 90.0) * T3 +
+         (7.0 * B4 * m2 * m2 * m2 - 4.0 * B6 * m2) / 1890.0 * T4;
+}
+
+static double periodicB(const wfl_geometry *g, double rho2) {
+  double a = g->a, rho = sqrt(rho2), lam = sqrt(g->l2);
+  double ex = exp(-rho / lam), flraf = floor(rho / a), ceilraf = ceil(rho / a), f;
+  if (rho < a) {
+    double eal = ((a - rho) / lam < 10.0) ? exp(-(rho - a) / lam) : 0.0;
+    f = ex / (a * (1.0 + ex) * (1.0 + ex));
+    if (rho > 0.0) f += eal / (rho * (1.0 + eal) * (1.0 + eal));
+  } else if (fabs(trunc(fmod(rho, a))) < 1.0e-6) {   /* integer abs, Integrand.c:79 */
+    f = flraf / (4.0 * rho);
+  } else {
+    double e1 = exp(-(rho - flraf * a) / lam), e2 = exp(-(rho - ceilraf * a) / lam);
+    f = flraf * e1 / (rho * (1.0 + e1) * (1.0 + e1)) + ceilraf * e2 / (rho * (1.0 + e2) * (1.0 + e2));
+  }
+  return a / (2.0 * log(2.0) * g->l2) * f;
+}
+
+static double fixfluxB(const wfl_geometry *g, double rho2) {
+  float l = (float)sqrt(g->l2);                               /* float locals, Integrand.c:51-52 */
+  float lml = (float)(((double)l - g->lam_min) / (g->a - g->lam_min));
+  double x = 2.0 * sqrt(rho2 / g->l2);
+  double bd = (fabs(x) < 1.0) ? exp(-1.0 / (1.0 - x * x)) : 0.0;
+  if (sqrt(rho2) > g->a / 2.0) printf("Warning: inappropriate magnetic field being used. rho=%f\n", sqrt(rho2));
+  return 2.0 / (g->l2 * g->fix_q2) * (1.0 - 0.75 * (double)lml) * bd + 3.0 / (g->a * g->a) * (double)lml;
+}
+
+static double magfield_geom(const wfl_geometry *g, double rho2) {
+  double B = 0.0, l2 = g->l2;
+  switch (g->kind) {
+    case WFL_STEP: B = rho2 < l2 ? 1.0 / l2 : 0.0; break;
+    case WFL_SMOOTH: B = l2 / ((l2 + rho2) * (l2 + rho2)); break;
+    case WFL_QUADRATIC: B = rho2 < l2 ? 2.0 / l2 * (1.0 - rho2 / l2) : 0.0; break;
+    case WFL_GAUSSIAN: B = rho2 < 100.0 * l2 ? 1.0 / (l2 * exp(rho2 / l2)) : 0.0; break;
+    case WFL_PERIODIC: B = periodicB(g, rho2); break;
+    case WFL_SPLINE: {
+      double s = sin(WFL_PI * sqrt(rho2) / g->a);
+      B = 1.0 / (l2 * exp(s * s / l2));
+    } break;
+    case WFL_FIXFLUX:
+    case WFL_FIXFLUX_FF: B = fixfluxB(g, rho2); break;
+  }
+  return 2.0 * g->F / WFL_E * B;
+}
+
+double MagField(double rho2, void *p) {
+  struct Wparams *params = (struct Wparams *)p;
+  wfl_geometry g = g_prof;
+  g.F = params->F;
+  g.l2 = params->l2;
+  return magfield_geom(&g, rho2);
+}
+
+static double smallT_geom(const wfl_geometry *g, double rho) {
+  double l2 = g->l2, a = g->a;
+  switch (g->kind) {
+    case WFL_STEP: return fabs(trunc(rho * rho - l2));          /* integer abs, Integrand.c:222 */
+    case WFL_SMOOTH:
+    case WFL_QUADRATIC:
+    case WFL_PERIODIC: return 0.5 * (l2 + rho * rho);
+    case WFL_GAUSSIAN: return l2;
+    case WFL_SPLINE: {
+      double s = sin(2.0 * WFL_PI * rho / a);
+      return (rho / a > 1.0e-8) ? a * a * l2 / (WFL_PI * WFL_PI * s * s) : 1.0e12;
+    }
+    case WFL_FIXFLUX:
+    case WFL_FIXFLUX_FF:
+      if (rho * rho < l2 / 4.0) return 0.25 * l2;
+      return 0.25 * l2 + (rho - sqrt(l2) / 2.0) * (rho - sqrt(l2) / 2.0);
+  }
+  return l2;
+}
+
+/* The integrand of one group at T given <W> - 1 (Integrand.c:252-305).
+ * Returns the branch taken: 0 small-T series, 1 constant-field closed form,
+ * 2 large-T asymptote, 3 worldline estimate. */
+static double igrand_geom(const wfl_geometry *g, int fermion, double T, double rho, double rho2,
+                          double evm1, int *branch) {
+  double B = magfield_geom(g, rho2), TB = WFL_E * T * B, sT = smallT_geom(g, rho);
+  if (T < 0.1 && T < 0.01 * sT) { *branch = 0; return fermion == 1 ? smTferm(B, T) : smTscal(B, T); }
+  if (T < 0.01 * sT) { *branch = 1; return TB > 1.0e-6 ? Exact(T, B, fermion) : 0.0; }
+  if (T > 49.0) {
+    *branch = 2;
+    if (fermion == 1) return -1.0 * exp(-WFL_M2 * T) / (3.0 * T) * WFL_E * WFL_E * B * B;
+    return exp(-WFL_M2 * T) / (6.0 * T) * WFL_E * WFL_E * B * B;
+  }
+  *branch = 3;
+  if (fermion == 1) return exp(-WFL_M2 * T) / (T * T * T) * (evm1 - 1.0 / 3.0 * TB * TB);
+  return exp(-WFL_M2 * T) / (T * T * T) * (evm1 + 1.0 / 6.0 * TB * TB);
+}
+
+double *Tfunc(double T, void *p, int *order) {
+  struct Wparams *params = (struct Wparams *)p;
+  float rho2f = params->xcm.x * params->xcm.x + params->xcm.y * params->xcm.y;   /* float, Integrand.c:199 */
+  double rho2 = rho2f;
+  double rho = params->xcm.x;
+  const int groupsize = 128;
+  double *igrand = (double *)malloc(sizeof(double) * (size_t)params->ng);
+  int *WLlist = (int *)malloc(sizeof(int) * groupsize);
+  if (!igrand || !WLlist) { printf("error Tfunc():  malloc failed\n"); free(igrand); free(WLlist); return NULL; }
+  wfl_geometry g = g_prof;
+  g.F = params->F;
+  g.l2 = params->l2;
+  for (int i = 0; i < params->ng; ++i) {
+    int branch;
+    double v = igrand_geom(&g, params->fermion, T, rho, rho2, 0.0, &branch);
+    if (branch == 3) {
+      for (int j = 0; j < groupsize; ++j) WLlist[j] = order[j + i * groupsize];
+      double ev = EV(T, p, WLlist);
+      v = igrand_geom(&g, params->fermion, T, rho, rho2, ev - 1.0, &branch);
+    }
+    igrand[i] = v;
+  }
+  free(WLlist);
+  return igrand;
+}
+
+static double *transformT(double t, void *p, int *order) {    /* Tint.c:15-33 */
+  struct Wparams *params = (struct Wparams *)p;
+  double *tf = Tfunc((1.0 - t) / t, p, order);
+  if (!tf) return NULL;
+  for (int i = 0; i < params->ng; ++i) tf[i] = (tf[i] / t) / t;
+  return tf;
+}
+
+double *integrateT(int n, void *p) {    /* Tint.c:100-175 */
+  struct Wparams *params = (struct Wparams *)p;
+  int ng = params->ng;
+  double *val = (double *)calloc((size_t)ng, sizeof(double));
+  int *order = (int *)malloc(sizeof(int) * (size_t)params->Nl);
+  if (!val || !order) { printf("integrate(): malloc failed\n"); free(val); free(order); return NULL; }
+  for (int i = 0; i < params->Nl; ++i) order[i] = i;   /* shuffle disabled, Tint.c:92 */
+  double h = 1.0 / (double)n;
+  for (int inx = 0; inx < n; ++inx) {
+    printf("T integrate inx=%d\n", inx);
+    double t = h * ((double)inx + 1.0);
+    double *a = transformT(t, p, order);
+    double *b = transformT(h * ((double)inx + 0.5), p, order);
+    double *c = transformT(h * ((double)inx + 1.0), p, order);
+    if (!a || !b || !c) { free(a); free(b); free(c); free(order); free(val); return NULL; }
+    double avg = 0.0;
+    for (int ig = 0; ig < ng; ++ig) {
+      if (isinf(a[ig])) { printf("Warning: trt[%d]=%f.  Setting to zero.\n", ig, a[ig]); a[ig] = 0.0; }
+      if (isinf(b[ig])) { printf("Warning: trtphalf[%d]=%f.  Setting to zero.\n", ig, b[ig]); b[ig] = 0.0; }
+      if (isinf(c[ig])) { printf("Warning: trtp1[%d]=%f.  Setting to zero.\n", ig, c[ig]); c[ig] = 0.0; }
+      val[ig] += h / 6.0 * (a[ig] + 4.0 * b[ig] + c[ig]);
+      if (isnan(val[ig])) printf("for Tt=%f val[%d]=nan\n", t, ig);
+      avg += val[ig];
+    }
+    printf("mean for Tt=%f, %f\n", t, avg / ng);
+    free(a); free(b); free(c);
+  }
+  double avg = 0.0, sem = 0.0;
+  for (int i = 0; i < ng; ++i) avg += val[i];
+  avg /= ng;
+  for (int i = 0; i < ng; ++i) sem += (val[i] - avg) * (val[i] - avg);
+  sem = sqrt(sem / ((double)ng * ((double)ng - 1.0)));
+  if (params->fermion == 1) printf("Leffvsrho: %e %e %e\n", params->xcm.x, avg / (4.0 * WFL_PI), sem / (4.0 * WFL_PI));
+  else printf("Leffvsrho: %e %e %e\n", params->xcm.x, avg / (-2.0 * WFL_PI), sem / (2.0 * WFL_PI));
+  free(order);
+  return val;
+}
+
+/* ------------------------------------------------------------------ */
+/* Log-spaced T grids (the configs' proper-time grids)                  */
+/* ------------------------------------------------------------------ */
+double wfl_logsimpson(const double *T, const double *f, int n) {
+  if (n < 2) return 0.0;
+  double h = (log(T[n - 1]) - log(T[0])) / (double)(n - 1);
+  int m = n - 1;
+  if (m == 1) return 0.5 * h * (T[0] * f[0] + T[1] * f[1]);
+  int ms = (m % 2 == 0) ? m : m - 3;
+  double s = 0.0;
+  for (int i = 0; i + 2 <= ms; i += 2) s += h / 3.0 * (T[i] * f[i] + 4.0 * T[i + 1] * f[i + 1] + T[i + 2] * f[i + 2]);
+  if (ms != m)
+    s += 3.0 * h / 8.0 * (T[ms] * f[ms] + 3.0 * T[ms + 1] * f[ms + 1] + 3.0 * T[ms + 2] * f[ms + 2] + T[ms + 3] * f[ms + 3]);
+  return s;
+}
+
+/* L_eff(rho) per loop group from batched group sums on a log T grid.
+ * gsum: [nT][ngroups][3] sums of W - 1 (the tuples of one (geometry, x_cm)
+ * in T order); counts: loops per group.  policy 1 applies the reference's
+ * analytic small-/large-T branches, policy 0 uses the worldline estimate at
+ * every node.  T must be the effective proper times the kernels used. */
+int wfl_leff_groups(const wfl_geometry *g, int fermion, int nT, const double *T, double cx, double cy,
+                    const double *gsum, const int *counts, int ngroups, int policy, double *leff) {
+  if (!g || !T || !gsum || !counts || !leff || nT < 2 || ngroups < 1) { wfl_set_error("wfl_leff_groups: bad arguments"); return WFL_ERR_ARG; }
+  wfl_geometry gq = *g;
+  gq.F = wfl_q32(g->F);
+  gq.l2 = wfl_q32(g->l2);
+  double rho2 = cx * cx + cy * cy, rho = sqrt(rho2);
+  double *f = (double *)malloc(sizeof(double) * (size_t)nT);
+  if (!f) { wfl_set_error("wfl_leff_groups: malloc"); return WFL_ERR_NOMEM; }
+  for (int q = 0; q < ngroups; ++q) {
+    for (int k = 0; k < nT; ++k) {
+      int branch;
+      double evm1 = wfl_group_ev_minus1(gsum + ((size_t)k * ngroups + q) * 3, counts[q]);
+      double v = igrand_geom(&gq, fermion, T[k], rho, rho2, evm1, &branch);
+      if (!policy && branch != 3) {
+        double B = magfield_geom(&gq, rho2), TB = WFL_E * T[k] * B;
+        v = (fermion == 1) ? exp(-WFL_M2 * T[k]) / (T[k] * T[k] * T[k]) * (evm1 - TB * TB / 3.0)
+                           : exp(-WFL_M2 * T[k]) / (T[k] * T[k] * T[k]) * (evm1 + TB * TB / 6.0);
+      }
+      f[k] = v;
+    }
+    double I = wfl_logsimpson(T, f, nT);
+    leff[q] = (fermion == 1) ? I / (4.0 * WFL_PI) : I / (-2.0 * WFL_PI);
+  }
+  free(f);
+  return WFL_OK;
+}
+
+double wfl_effective_T(double T) {
+  double r = (double)(float)sqrt(T);
+  return r * r;
+}
+
+/* ------------------------------------------------------------------ */
+/* Loops: generator and text readers                                    */
+/* ------------------------------------------------------------------ */
+static void philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]}, k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    c[1] = (uint32_t)p1; c[3] = (uint32_t)p0; c[0] = n0; c[2] = n2;
+  }
+  memcpy(out, c, sizeof c);
+}
+
+int wfl_gen_loops(uint32_t config_id, int Nl, int N, float *x, float *y) {
+  if (!x || !y || Nl < 1 || N < 4 || N % 4) { wfl_set_error("wfl_gen_loops: bad arguments"); return WFL_ERR_ARG; }
+  const double sigma = sqrt(2.0 / (double)N), two_pi = 2.0 * WFL_PI, inv32 = 1.0 / 4294967296.0;
+  int fail = 0;
+#pragma omp parallel
+  {
+    double *dx = (double *)malloc(sizeof(double) * (size_t)N), *dy = (double *)malloc(sizeof(double) * (size_t)N);
+    if (!dx || !dy) {
+#pragma omp atomic write
+      fail = 1;
+    } else {
+#pragma omp for schedule(static)
+      for (int l = 0; l < Nl; ++l) {
+        for (int j = 0; j < N / 2; ++j) {
+          uint32_t ctr[4] = {(uint32_t)j, (uint32_t)l, 0u, 0u}, key[2] = {0x5EED0000u + config_id, 0u}, r[4];
+          philox4x32_10(ctr, key, r);
+          double ra = sqrt(-2.0 * log(((double)r[0] + 0.5) * inv32)), aa = two_pi * (((double)r[1] + 0.5) * inv32);
+          double rb = sqrt(-2.0 * log(((double)r[2] + 0.5) * inv32)), ab = two_pi * (((double)r[3] + 0.5) * inv32);
+          dx[2 * j] = sigma * ra * cos(aa);
+          dy[2 * j] = sigma * ra * sin(aa);
+          dx[2 * j + 1] = sigma * rb * cos(ab);
+          dy[2 * j + 1] = sigma * rb * sin(ab);
+        }
+        double mx = 0.0, my = 0.0;
+        for (int i = 0; i < N; ++i) { mx += dx[i]; my += dy[i]; }
+        mx /= N; my /= N;
+        double px = 0.0, py = 0.0, cx = 0.0, cy = 0.0;
+        for (int i = 0; i < N; ++i) {
+          px += dx[i] - mx; py += dy[i] - my;
+          dx[i] = px; dy[i] = py;
+          cx += px; cy += py;
+        }
+        cx /= N; cy /= N;
+        for (int i = 0; i < N; ++i) {
+          x[(size_t)l * N + i] = (float)(dx[i] - cx);
+          y[(size_t)l * N + i] = (float)(dy[i] - cy);
+        }
+      }
+    }
+    free(dx); free(dy);
+  }
+  if (fail) { wfl_set_error("wfl_gen_loops: out of memory"); return WFL_ERR_NOMEM; }
+  return WFL_OK;
+}
+
+int wfl_read_text_loops(const char *path, int Nl, int N, float *x, float *y) {
+  if (!path || !x || !y || Nl < 1 || N < 1) { wfl_set_error("wfl_read_text_loops: bad arguments"); return WFL_ERR_ARG; }
+  FILE *fp = fopen(path, "rb");
+  if (!fp) { wfl_set_error("cannot open %s", path); return WFL_ERR_ARG; }
+  for (int l = 0; l < Nl; ++l) {
+    for (int j = 0; j < N; ++j)
+      if (fscanf(fp, "%f ", &x[(size_t)l * N + j]) != 1) { fclose(fp); wfl_set_error("%s: short read at loop %d (x)", path, l); return WFL_ERR_ARG; }
+    for (int j = 0; j < N; ++j)
+      if (fscanf(fp, "%f ", &y[(size_t)l * N + j]) != 1) { fclose(fp); wfl_set_error("%s: short read at loop %d (y)", path, l); return WFL_ERR_ARG; }
+  }
+  fclose(fp);
+  return WFL_OK;
+}
+
+int getwl(float4 *worldlines, char *filename, int Nl, int Nppl, int Dim) {   /* getwl.c:6-40 */
+  printf("getting worldlines from %s\n", filename);
+  FILE *fp = fopen(filename, "rb");
+  if (!fp) { printf("cannot open %s\n", filename); return 0; }
+  for (int i = 0; i < Nl; ++i) {
+    for (int j = 0; j < Nppl; ++j)
+      if (fscanf(fp, "%f ", &worldlines[(size_t)i * Nppl + j].x) != 1) break;
+    if (Dim > 1) {
+      for (int j = 0; j < Nppl; ++j)
+        if (fscanf(fp, "%f ", &worldlines[(size_t)i * Nppl + j].y) != 1) break;
+      if (Dim > 2)
+        for (int j = 0; j < Nppl; ++j)
+          if (fscanf(fp, "%f ", &worldlines[(size_t)i * Nppl + j].z) != 1) break;
+    }
+  }
+  fclose(fp);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Spline coefficients (Burden & Faires Alg. 3.5, clamped, zero slopes) */
+/* ------------------------------------------------------------------ */
+int wfl_spline_build(const double *fli, const float *x, int n, int quirk, float *c4) {
+  if (!fli || !x || !c4 || n < 3) { wfl_set_error("wfl_spline_build: need n >= 3 and buffers"); return WFL_ERR_ARG; }
+  double *w = (double *)malloc(sizeof(double) * 6 * (size_t)n);
+  if (!w) { wfl_set_error("wfl_spline_build: malloc"); return WFL_ERR_NOMEM; }
+  double *h = w, *l = w + n, *mu = w + 2 * n, *z = w + 3 * n, *al = w + 4 * n, *c = w + 5 * n;
+  for (int i = 0; i < n - 1; ++i) {
+    h[i] = quirk ? (double)(x[i + 1] - x[i]) : (double)x[i + 1] - (double)x[i];
+    if (!(h[i] > 0.0)) { free(w); wfl_set_error("wfl_spline_build: knots must increase"); return WFL_ERR_ARG; }
+  }
+  al[0] = 3.0 * (fli[1] - fli[0]) / h[0];
+  al[n - 1] = -3.0 * (fli[n - 1] - fli[n - 2]) / h[n - 2];
+  for (int i = 1; i < n - 1; ++i) al[i] = 3.0 / h[i] * (fli[i + 1] - fli[i]) - 3.0 / h[i - 1] * (fli[i] - fli[i - 1]);
+  l[0] = 2.0 * h[0]; mu[0] = 0.5; z[0] = al[0] / l[0];
+  for (int i = 1; i < n - 1; ++i) {
+    double span = quirk ? (double)(x[i + 1] - x[i - 1]) : (double)x[i + 1] - (double)x[i - 1];
+    l[i] = 2.0 * span - h[i - 1] * mu[i - 1];
+    mu[i] = h[i] / l[i];
+    z[i] = (al[i] - h[i - 1] * z[i - 1]) / l[i];
+  }
+  l[n - 1] = h[n - 2] * (2.0 - mu[n - 2]);
+  z[n - 1] = (al[n - 1] - h[n - 2] * z[n - 2]) / l[n - 1];
+  c[n - 1] = quirk ? (double)(float)z[n - 1] : z[n - 1];
+  memset(c4, 0, sizeof(float) * 4 * (size_t)n);
+  for (int i = n - 2; i >= 0; --i) {
+    double ci = z[i] - mu[i] * c[i + 1];
+    c[i] = quirk ? (double)(float)ci : ci;
+    double bi = (fli[i + 1] - fli[i]) / h[i] - h[i] * (c[i + 1] + 2.0 * c[i]) / 3.0;
+    double di = quirk ? (double)((float)c[i + 1] - (float)c[i]) / (3.0 * h[i]) : (c[i + 1] - c[i]) / (3.0 * h[i]);
+    c4[4 * i] = (float)fli[i]; c4[4 * i + 1] = (float)bi; c4[4 * i + 2] = (float)c[i]; c4[4 * i + 3] = (float)di;
+  }
+  c4[4 * (n - 1)] = (float)fli[n - 1];
+  c4[4 * (n - 1) + 2] = (float)c[n - 1];
+  free(w);
+  return WFL_OK;
+}

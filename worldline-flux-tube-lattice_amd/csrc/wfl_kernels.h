@@ -1,0 +1,378 @@
+// wfl_kernels.h -- the Wilson-loop kernels for gfx950 (templated on the
+// arithmetic type; instantiated in wfl_kernels_f32.hip / wfl_kernels_f64.hip).
+//
+// Data: each loop point i carries a record of six values
+//   (w_x, w_y, dw_x, dw_y, e_x, e_y),  dw = w_succ(i) - w_i,  e = w_succ(i) - w_pred(i)
+// where succ/pred follow the sub-loop map of getzp1 (intEV.cu:462-506).
+// Records are stored point-major inside each loop ([Nl][N][6]) so that one
+// wavefront walking one loop reads them with scalar loads.
+//
+// Two mappings of work to the machine:
+//  * k_tuple (lane per tuple): the 64 lanes of a wavefront evaluate 64
+//    (geometry, x_cm, T) tuples on the SAME loop point at a time; the point
+//    record is wave-uniform (SGPRs), nothing is shared between lanes, and
+//    each lane reduces its own tuple's group sums in FP64 registers.
+//  * k_loop (wave per loop): one wavefront evaluates one (tuple, loop) pair,
+//    lanes stride over points; sub-loop sums are combined with a 4-step
+//    cross-lane butterfly.  Used for EV() and for batches with few tuples.
+//
+// Per point i (sub-loop class c: odd -> x, 2 mod 4 -> y, 0 mod 4 -> z) with
+// z = c_img + s w_i, s = sqrt(T):
+//   PHYSICAL:  P_c += (z x e) g(|z|^2) + (z x dw) sum_k w_k g(|z + t_k s dw|^2)
+//   REFERENCE: P_c += (z x dw) [g(A) + g(A1) + sum_k w_k g(rho^2(t_k))]  (clamped A, B, C)
+// with Simpson weights w_k = 4 (odd k), 2 (even k), t_k = k/n, and
+// Phi_c = F s (h/3) P_c, h = 1/n.  The spin factor uses f' the same way.
+#pragma once
+#include "wfl_profiles.h"
+
+namespace wfl {
+
+struct Task { int lanegroup, loop0, nloop, group; };
+
+__device__ __forceinline__ int succ_idx(int i, int N) {
+  if (i & 1) return (i == N - 1) ? 1 : i + 2;
+  if ((i & 3) == 0) return (i == N - 4) ? 0 : i + 4;
+  return (i == N - 2) ? 2 : i + 4;
+}
+__device__ __forceinline__ int pred_idx(int i, int N) {
+  if (i & 1) return (i == 1) ? N - 1 : i - 2;
+  if ((i & 3) == 0) return (i == 0) ? N - 4 : i - 4;
+  return (i == 2) ? N - 2 : i - 4;
+}
+// accumulator slot of point i: 0 = x (odd), 1 = y (2 mod 4), 2 = z (0 mod 4)
+__device__ __forceinline__ int cls_of(int i) { return (i & 1) ? 0 : ((i & 3) == 0 ? 2 : 1); }
+
+// Per-lane constants of one tuple in the kernel's type.
+template <class R> struct LaneC {
+  R s, hs;                 // sqrt(T), sqrt(T)/2
+  R inv_l2;
+  float k_hi, k_lo;
+  R spin_scale;            // 1/l2 on the Gaussian fast path (f' = e/l2), else 1
+};
+
+// --------------------------------------------------------- node evaluation
+template <class R, int PROF>
+__device__ __forceinline__ R node_g(R u, const LaneC<R>& lc, int kind, const GP<R>& gp, int mode,
+                                    const float* __restrict__ kn, const float4* __restrict__ cf, int ns,
+                                    bool spinor, R& fp) {
+  if (PROF == P_GAUSSIAN) {
+    R e;
+    R gval = gauss_g(u, lc.inv_l2, lc.k_hi, lc.k_lo, e);
+    fp = e;
+    return gval;
+  } else {
+    return generic_g<R>(kind, gp, u, mode, spinor, kn, cf, ns, fp);
+  }
+}
+
+// One point (= one segment start) for one image centre.  Adds to P (phase)
+// and S (spin) of the point's class; all terms are before the common
+// factors F s h/3 (phase) and h/3 * spin_scale (spin).
+template <class R, int PROF, int SPINOR, int MODE, int NSUB>
+__device__ __forceinline__ void point_eval(R wx, R wy, R dx, R dy, R ex, R ey,
+                                           R ccx, R ccy, const LaneC<R>& lc, int kind, const GP<R>& gp,
+                                           const float* __restrict__ kn, const float4* __restrict__ cf, int ns,
+                                           int nsub_rt, R& P, R& S) {
+  const int n = NSUB ? NSUB : nsub_rt;
+  R zx = fma(lc.s, wx, ccx);
+  R zy = fma(lc.s, wy, ccy);
+  R X = fma(zx, dy, -(zy * dx));          // z x dw   (z x z' = s (z x dw))
+  if (MODE == M_PHYSICAL) {
+    R E = fma(zx, ey, -(zy * ex));        // z x e    (z x (z'-z_prev) / s)
+    R A = fma(zx, zx, zy * zy);
+    R fpA, fpk;
+    R gA = node_g<R, PROF>(A, lc, kind, gp, MODE, kn, cf, ns, SPINOR, fpA);
+    R gin = (R)0, fin = (R)0;
+    if (n == 2) {
+      R mx = fma(lc.hs, dx, zx), my = fma(lc.hs, dy, zy);
+      R Am = fma(mx, mx, my * my);
+      gin = (R)4 * node_g<R, PROF>(Am, lc, kind, gp, MODE, kn, cf, ns, SPINOR, fpk);
+      fin = (R)4 * fpk;
+    } else {
+      const R h = (R)1 / (R)n;
+      for (int k = 1; k < n; ++k) {
+        R t = (R)k * h * lc.s;
+        R px = fma(t, dx, zx), py = fma(t, dy, zy);
+        R u = fma(px, px, py * py);
+        R w = (k & 1) ? (R)4 : (R)2;
+        R gk = node_g<R, PROF>(u, lc, kind, gp, MODE, kn, cf, ns, SPINOR, fpk);
+        gin = fma(w, gk, gin);
+        if (SPINOR) fin = fma(w, fpk, fin);
+      }
+    }
+    P = fma(E, gA, fma(X, gin, P));
+    if (SPINOR) S += fma((R)2, fpA, fin);
+  } else {
+    // Reference Idt: rho^2(t) = A + 2Bt + Ct^2 with the clamps of intEV.cu:418-422, 439.
+    R dsx = lc.s * dx, dsy = lc.s * dy;
+    R A = fma(zx, zx, zy * zy);
+    R B = fma(zx, dsx, zy * dsy);
+    R C = fma(dsx, dsx, dsy * dsy);
+    const R c8 = (R)1.0e-8, c10 = (R)1.0e-10;
+    A = A < c8 ? c8 : A;
+    B = B < c8 ? c8 : B;
+    C = C < c8 ? c8 : C;
+    R A1 = A + (R)2 * B + C;
+    A1 = A1 < c8 ? c8 : A1;
+    R f0, f1, fk;
+    R g0 = node_g<R, PROF>(A, lc, kind, gp, MODE, kn, cf, ns, SPINOR, f0);
+    R g1 = node_g<R, PROF>(A1, lc, kind, gp, MODE, kn, cf, ns, SPINOR, f1);
+    R gs = g0 + g1, fs = SPINOR ? f0 + f1 : (R)0;
+    if (n == 2) {
+      R u = A + B + (R)0.25 * C;
+      u = u < c10 ? c10 : u;
+      gs = fma((R)4, node_g<R, PROF>(u, lc, kind, gp, MODE, kn, cf, ns, SPINOR, fk), gs);
+      if (SPINOR) fs = fma((R)4, fk, fs);
+    } else {
+      const R h = (R)1 / (R)n;
+      for (int k = 1; k < n; ++k) {
+        R t = (R)k * h;
+        R u = A + (R)2 * B * t + C * t * t;
+        u = u < c10 ? c10 : u;
+        R w = (k & 1) ? (R)4 : (R)2;
+        gs = fma(w, node_g<R, PROF>(u, lc, kind, gp, MODE, kn, cf, ns, SPINOR, fk), gs);
+        if (SPINOR) fs = fma(w, fk, fs);
+      }
+    }
+    P = fma(X, gs, P);
+    if (SPINOR) S += fs;
+  }
+}
+
+template <class R> __device__ __forceinline__ bool finite_r(R v) { return v - v == (R)0; }
+
+// W - 1 of the three Richardson estimates from the per-loop sums
+// (Integrand.c:352-390), in FP64; the reference's cosh guard
+// (Integrand.c:371-377, |trunc(arg)| > 600) in reference mode.
+__device__ __forceinline__ bool guard600(double v) {
+  if (!(fabs(v) < 2147483648.0)) return false;
+  return fabs(trunc(v)) > 600.0;
+}
+__device__ __forceinline__ void wminus1(int mode, bool spinor, const double sc[3], const double fe[3], double wm[3]) {
+  double yz = sc[1] + sc[2];
+  double a[3] = {0.5 * sc[0] + 0.25 * yz, sc[0], 0.5 * yz};
+  if (!spinor) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { double s = sin(0.5 * a[k]); wm[k] = -2.0 * s * s; }
+    return;
+  }
+  double fyz = fe[1] + fe[2];
+  double b[3] = {0.5 * fe[0] + 0.25 * fyz, fe[0], 0.5 * fyz};
+  if (mode == M_REFERENCE && (guard600(fyz) || guard600(fe[0]))) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) wm[k] = cos(a[k]) * 1.0e10 - 1.0;
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double s = sin(0.5 * a[k]), sh = sinh(0.5 * b[k]);
+    wm[k] = 2.0 * sh * sh - 2.0 * s * s * cosh(b[k]);
+  }
+}
+
+template <class R>
+__device__ __forceinline__ void lane_setup(const DevTuple& tp, const DevGeom& gm, int PROF, LaneC<R>& lc) {
+  lc.s = (R)tp.rtT;
+  lc.hs = (R)0.5 * lc.s;
+  lc.inv_l2 = sizeof(R) == 4 ? (R)gm.inv_l2f : (R)gm.inv_l2;
+  lc.k_hi = gm.k_hi;
+  lc.k_lo = gm.k_lo;
+  lc.spin_scale = (PROF == P_GAUSSIAN) ? lc.inv_l2 : (R)1;
+}
+
+// Per-loop outputs from the class sums (FP64).
+__device__ __forceinline__ void finalize_loop(const double P[3], const double S[3], double F, double s,
+                                              double spin_scale, int n, int N, bool spinor,
+                                              double sc[3], double fe[3]) {
+  const double h3 = 1.0 / (3.0 * (double)n);
+  sc[0] = F * s * h3 * P[0];
+  sc[1] = F * s * h3 * P[1];
+  sc[2] = F * s * h3 * P[2];
+  if (spinor) {
+    const double k = 2.0 * F * s * s * h3 * spin_scale;
+    fe[0] = k * S[0] / (0.5 * N);
+    fe[1] = k * S[1] / (0.25 * N);
+    fe[2] = k * S[2] / (0.25 * N);
+  } else {
+    fe[0] = fe[1] = fe[2] = 0.0;
+  }
+}
+
+// ------------------------------------------------------ lane per tuple
+template <class R, int PROF, int SPINOR, int MODE, int NSUB>
+__global__ __launch_bounds__(256) void k_tuple(
+    const R* __restrict__ rec, int N, int Nl, const DevTuple* __restrict__ tups,
+    const DevGeom* __restrict__ geoms, const int* __restrict__ lanes,
+    const float* __restrict__ knots, const float4* __restrict__ coefs,
+    const Task* __restrict__ tasks, int ntasks, int ngroups, int nsub_rt,
+    double* __restrict__ gsum, double* __restrict__ perloop,
+    unsigned long long* __restrict__ nonfinite) {
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int task_id = blockIdx.x * 4 + wv;
+  if (task_id >= ntasks) return;
+  const int lane = threadIdx.x & 63;
+  const Task tk = tasks[task_id];
+  const int lg = __builtin_amdgcn_readfirstlane(tk.lanegroup);
+  const int loop0 = __builtin_amdgcn_readfirstlane(tk.loop0);
+  const int nloop = __builtin_amdgcn_readfirstlane(tk.nloop);
+  const int grp = __builtin_amdgcn_readfirstlane(tk.group);
+  const int tid_raw = lanes[lg * 64 + lane];
+  const bool valid = tid_raw >= 0;
+  const int tid = valid ? tid_raw : lanes[lg * 64];
+  const DevTuple tp = tups[tid];
+  const DevGeom gm = geoms[tp.geom];
+  const GP<R> gp(gm);
+  LaneC<R> lc;
+  lane_setup<R>(tp, gm, PROF, lc);
+  const int kind = gm.kind;
+  const float* kn = knots + gm.spline_off;
+  const float4* cf = coefs + gm.spline_off;
+  const int ns = gm.spline_n;
+  const int n = NSUB ? NSUB : nsub_rt;
+  // image window (uniform over the lane group: one geometry per group)
+  const int M = __builtin_amdgcn_readfirstlane(gm.lattice > 0.0 ? gm.images : 0);
+  const R lat = sizeof(R) == 4 ? (R)gm.latticef : (R)gm.lattice;
+  const R cx = (R)tp.cx, cy = (R)tp.cy;
+
+  double G[3] = {0.0, 0.0, 0.0};
+  for (int l = loop0; l < loop0 + nloop; ++l) {
+    const R* __restrict__ rl = rec + (size_t)l * N * 6;
+    double P[3] = {0.0, 0.0, 0.0}, S[3] = {0.0, 0.0, 0.0};
+    for (int ia = -M; ia <= M; ++ia) {
+      for (int ib = -M; ib <= M; ++ib) {
+        R ccx = cx, ccy = cy;
+        if (M > 0 || gm.lattice > 0.0) {
+          ccx = fma(-(R)(tp.m0 + ia), lat, cx);
+          ccy = fma(-(R)(tp.n0 + ib), lat, cy);
+        }
+        for (int i0 = 0; i0 < N; i0 += 32) {
+          R p[3] = {(R)0, (R)0, (R)0}, q[3] = {(R)0, (R)0, (R)0};
+          const int i1 = i0 + 32 < N ? i0 + 32 : N;
+          for (int i = i0; i < i1; i += 4) {
+            const R* r = rl + (size_t)i * 6;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const int c = (u & 1) ? 0 : (u == 0 ? 2 : 1);
+              point_eval<R, PROF, SPINOR, MODE, NSUB>(r[6 * u + 0], r[6 * u + 1], r[6 * u + 2], r[6 * u + 3],
+                                                      r[6 * u + 4], r[6 * u + 5], ccx, ccy, lc, kind, gp, kn, cf,
+                                                      ns, nsub_rt, p[c], q[c]);
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < 3; ++c) { P[c] += (double)p[c]; if (SPINOR) S[c] += (double)q[c]; }
+        }
+      }
+    }
+    double sc[3], fe[3], wm[3];
+    finalize_loop(P, S, gm.F, (double)tp.rtT, (double)lc.spin_scale, n, N, SPINOR, sc, fe);
+    wminus1(MODE, SPINOR, sc, fe, wm);
+    bool ok = finite_r(wm[0]) && finite_r(wm[1]) && finite_r(wm[2]);
+    if (valid && !ok) atomicAdd(nonfinite, 1ull);
+    G[0] += wm[0]; G[1] += wm[1]; G[2] += wm[2];
+    if (perloop && valid) {
+      double* o = perloop + ((size_t)tid * Nl + l) * 6;
+      o[0] = sc[0]; o[1] = sc[1]; o[2] = sc[2]; o[3] = fe[0]; o[4] = fe[1]; o[5] = fe[2];
+    }
+  }
+  if (valid) {
+    double* o = gsum + ((size_t)tid * ngroups + grp) * 3;
+    o[0] = G[0]; o[1] = G[1]; o[2] = G[2];
+  }
+}
+
+// ------------------------------------------------------- wave per loop
+__device__ __forceinline__ double shfl_xor_d(double v, int m) {
+  return __shfl_xor(v, m, 64);
+}
+
+template <class R, int PROF, int SPINOR, int MODE, int NSUB>
+__global__ __launch_bounds__(256) void k_loop(
+    const R* __restrict__ rec, int N, int Nl, const DevTuple* __restrict__ tups,
+    const DevGeom* __restrict__ geoms, int ntup,
+    const float* __restrict__ knots, const float4* __restrict__ coefs, int nsub_rt,
+    double* __restrict__ perloop, double* __restrict__ wm1,
+    float4* __restrict__ ws_scal, float4* __restrict__ ws_ferm,
+    unsigned long long* __restrict__ nonfinite) {
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long w = (long long)blockIdx.x * 4 + wv;
+  if (w >= (long long)ntup * Nl) return;
+  const int t = (int)(w / Nl), l = (int)(w % Nl);
+  const int lane = threadIdx.x & 63;
+  const DevTuple tp = tups[t];
+  if (l >= tp.nloops) return;
+  const DevGeom gm = geoms[tp.geom];
+  const GP<R> gp(gm);
+  LaneC<R> lc;
+  lane_setup<R>(tp, gm, PROF, lc);
+  const int kind = gm.kind;
+  const float* kn = knots + gm.spline_off;
+  const float4* cf = coefs + gm.spline_off;
+  const int ns = gm.spline_n;
+  const int n = NSUB ? NSUB : nsub_rt;
+  const int M = gm.lattice > 0.0 ? gm.images : 0;
+  const R lat = sizeof(R) == 4 ? (R)gm.latticef : (R)gm.lattice;
+  const R cx = (R)tp.cx, cy = (R)tp.cy;
+  const R* __restrict__ rl = rec + (size_t)l * N * 6;
+
+  double P = 0.0, S = 0.0;      // this lane's class: lane & 3 (points i = lane + 64k)
+  for (int ia = -M; ia <= M; ++ia) {
+    for (int ib = -M; ib <= M; ++ib) {
+      R ccx = cx, ccy = cy;
+      if (gm.lattice > 0.0) {
+        ccx = fma(-(R)(tp.m0 + ia), lat, cx);
+        ccy = fma(-(R)(tp.n0 + ib), lat, cy);
+      }
+      for (int i0 = lane; i0 < N; i0 += 64 * 8) {
+        R p = (R)0, q = (R)0;
+        const int i1 = i0 + 64 * 8 < N ? i0 + 64 * 8 : N;
+        for (int i = i0; i < i1; i += 64) {
+          const R* r = rl + (size_t)i * 6;
+          point_eval<R, PROF, SPINOR, MODE, NSUB>(r[0], r[1], r[2], r[3], r[4], r[5], ccx, ccy, lc, kind, gp, kn,
+                                                  cf, ns, nsub_rt, p, q);
+        }
+        P += (double)p;
+        S += (double)q;
+      }
+    }
+  }
+  // lanes with equal (lane & 3) hold the same class: reduce over them
+#pragma unroll
+  for (int m = 4; m < 64; m <<= 1) {
+    P += shfl_xor_d(P, m);
+    if (SPINOR) S += shfl_xor_d(S, m);
+  }
+  // lane 0: class z (0 mod 4), lanes 1 and 3: class x (odd), lane 2: class y
+  double P1 = __shfl(P, 1, 64), P2 = __shfl(P, 2, 64), P3 = __shfl(P, 3, 64);
+  double S1 = __shfl(S, 1, 64), S2 = __shfl(S, 2, 64), S3 = __shfl(S, 3, 64);
+  if (lane == 0) {
+    double Pc[3] = {P1 + P3, P2, P};
+    double Sc[3] = {S1 + S3, S2, S};
+    double sc[3], fe[3], wm[3];
+    finalize_loop(Pc, Sc, gm.F, (double)tp.rtT, (double)lc.spin_scale, n, N, SPINOR, sc, fe);
+    wminus1(MODE, SPINOR, sc, fe, wm);
+    if (!(finite_r(wm[0]) && finite_r(wm[1]) && finite_r(wm[2]))) atomicAdd(nonfinite, 1ull);
+    const size_t o = (size_t)t * Nl + l;
+    if (perloop) {
+      double* po = perloop + o * 6;
+      po[0] = sc[0]; po[1] = sc[1]; po[2] = sc[2]; po[3] = fe[0]; po[4] = fe[1]; po[5] = fe[2];
+    }
+    if (wm1) { wm1[o * 3] = wm[0]; wm1[o * 3 + 1] = wm[1]; wm1[o * 3 + 2] = wm[2]; }
+    if (ws_scal) ws_scal[l] = make_float4((float)sc[0], (float)sc[1], (float)sc[2], 0.0f);
+    if (ws_ferm && SPINOR) ws_ferm[l] = make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.0f);
+  }
+}
+
+}  // namespace wfl
+
+// Launch interface of one precision's kernel instantiations.
+struct WflLaunch {
+  const void* rec; int N, Nl;
+  const wfl::DevTuple* tups; const wfl::DevGeom* geoms; int ntup;
+  const int* lanes; const wfl::Task* tasks; int ntasks; int ngroups;
+  const float* knots; const float4* coefs;
+  int prof_gauss, spinor, mode, nsub;
+  double* gsum; double* perloop; double* wm1;
+  float4* ws_scal; float4* ws_ferm;
+  unsigned long long* nonfinite;
+  hipStream_t stream;
+};

@@ -1,0 +1,319 @@
+// wfl_profiles.h -- device-side field profiles f(rho^2), f'(rho^2) and the
+// segment integrand g(u) = f(u)/u for gfx950, templated on the arithmetic
+// type (float for the mixed kernels, double for the FP64 kernels).
+//
+// Profiles follow intEV.cu:29-406 (flambda/fplambda and helpers) and the
+// intEVff.cu:134-243 variant; parameters are the FP32-quantized values the
+// host stores in DevGeom, so both precisions evaluate the same problem.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wfl {
+
+enum { P_STEP = 0, P_SMOOTH = 1, P_QUADRATIC = 2, P_GAUSSIAN = 3,
+       P_PERIODIC = 4, P_SPLINE = 5, P_FIXFLUX = 6, P_FIXFLUX_FF = 7 };
+enum { M_REFERENCE = 0, M_PHYSICAL = 1 };
+// kernel tag for "profile chosen at run time from DevGeom::kind"
+enum { P_ANY = 99 };
+
+// One geometry as the kernels see it.  Every double field holds a value
+// that is exactly representable in FP32.
+struct DevGeom {
+  int kind, images, spline_off, spline_n;
+  double F, l2, inv_l2, a, lmin, q1, q2, lattice;
+  float Ff, l2f, inv_l2f, af, lminf, q1f, q2f, latticef;
+  float k_hi, k_lo;        // -log2(e)/l2 split into hi + lo
+};
+
+// One evaluation tuple (x_cm, sqrt(T), geometry); the image-window centre
+// (m0, n0) is computed on the host in double.
+struct DevTuple {
+  int geom, m0, n0, nloops;
+  float cx, cy, rtT, pad;
+};
+
+// ---------------------------------------------------------------- math
+template <class R> struct Math;
+template <> struct Math<float> {
+  static __device__ __forceinline__ float exp2(float y) { return __builtin_amdgcn_exp2f(y); }
+  static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+  static __device__ __forceinline__ float exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+  static __device__ __forceinline__ float log(float x) { return __logf(x); }
+  static __device__ __forceinline__ float sqrt(float x) { return __builtin_sqrtf(x); }
+  static __device__ __forceinline__ float floor(float x) { return __builtin_floorf(x); }
+  static __device__ __forceinline__ float sin(float x) { return __sinf(x); }
+  static __device__ __forceinline__ float cos(float x) { return __cosf(x); }
+  static __device__ __forceinline__ float cosh(float x) { return ::coshf(x); }
+  static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+};
+template <> struct Math<double> {
+  static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
+  static __device__ __forceinline__ double log(double x) { return ::log(x); }
+  static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+  static __device__ __forceinline__ double floor(double x) { return ::floor(x); }
+  static __device__ __forceinline__ double sin(double x) { return ::sin(x); }
+  static __device__ __forceinline__ double cos(double x) { return ::cos(x); }
+  static __device__ __forceinline__ double cosh(double x) { return ::cosh(x); }
+  static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+};
+
+template <class R> struct GP;   // geometry parameters in the kernel's type
+template <> struct GP<float> {
+  float F, l2, inv_l2, a, lmin, q1, q2;
+  __device__ GP(const DevGeom& g) : F(g.Ff), l2(g.l2f), inv_l2(g.inv_l2f), a(g.af), lmin(g.lminf), q1(g.q1f), q2(g.q2f) {}
+};
+template <> struct GP<double> {
+  double F, l2, inv_l2, a, lmin, q1, q2;
+  __device__ GP(const DevGeom& g) : F(g.F), l2(g.l2), inv_l2(g.inv_l2), a(g.a), lmin(g.lmin), q1(g.q1), q2(g.q2) {}
+};
+
+// -------------------------------------------- Gaussian tube (hot profile)
+// phi1(x) = (1 - e^{-x})/x for x < 1/2 by its Taylor series (truncation
+// below 6e-10 relative); the quotient form elsewhere, where 1 - e^{-x}
+// keeps full relative precision.
+__device__ __forceinline__ float phi1_series(float x) {
+  float p = 1.0f / 362880.0f;
+  p = fmaf(p, -x, 1.0f / 40320.0f);
+  p = fmaf(p, -x, 1.0f / 5040.0f);
+  p = fmaf(p, -x, 1.0f / 720.0f);
+  p = fmaf(p, -x, 1.0f / 120.0f);
+  p = fmaf(p, -x, 1.0f / 24.0f);
+  p = fmaf(p, -x, 1.0f / 6.0f);
+  p = fmaf(p, -x, 0.5f);
+  p = fmaf(p, -x, 1.0f);
+  return p;
+}
+
+// FP32: returns g(u) = f(u)/u and e = exp(-u/l2).
+__device__ __forceinline__ float gauss_g(float u, float inv_l2, float k_hi, float k_lo, float& e) {
+  e = __builtin_amdgcn_exp2f(fmaf(u, k_hi, u * k_lo));
+  float x = u * inv_l2;
+  float q = (1.0f - e) * __builtin_amdgcn_rcpf(u);
+  return (x < 0.5f) ? phi1_series(x) * inv_l2 : q;
+}
+// FP64: g(u) = -expm1(-u/l2)/u (limit 1/l2 at u = 0), e = 1 + expm1(-u/l2).
+__device__ __forceinline__ double gauss_g(double u, double inv_l2, float, float, double& e) {
+  double x = u * inv_l2;
+  double em1 = ::expm1(-x);
+  e = 1.0 + em1;
+  return (u > 0.0) ? -em1 / u : inv_l2;
+}
+
+// ------------------------------------------------ remaining profiles
+template <class R>
+__device__ R expint_neg(R x) {   // Ei(x) = -E1(-x), x < -1 (intEV.cu:29-61), converged to R
+  const R EPS = sizeof(R) == 4 ? (R)1.0e-6 : (R)1.0e-15;
+  R b = -x + (R)1, c = (R)1e30, d = Math<R>::div((R)1, b), h = d;
+  for (int i = 1; i <= 1000; ++i) {
+    R a = -(R)i * (R)i;
+    b += (R)2;
+    d = Math<R>::div((R)1, a * d + b);
+    c = b + Math<R>::div(a, c);
+    R del = c * d;
+    h *= del;
+    if (fabs(del - (R)1) <= EPS) return -h * Math<R>::exp(x);
+  }
+  return (R)0;
+}
+
+template <class R> __device__ R bump(R x) {
+  return (x * x < (R)0.999) ? Math<R>::exp(Math<R>::div((R)-1, (R)1 - x * x)) : (R)0;
+}
+
+template <class R> __device__ R phi_fix(const GP<R>& g, R x, bool ff) {
+  R om = (R)1 - x * x;
+  if (x < (ff ? (R)1 : (R)0.999))
+    return (R)1 - Math<R>::div((R)0.5, g.q2) * (om * Math<R>::exp(Math<R>::div((R)-1, om)) + expint_neg<R>(Math<R>::div((R)-1, om)));
+  return (R)1;
+}
+
+template <class R> __device__ R chi_fix(const GP<R>& g, R x, R n, R lam, bool ff) {
+  R om = (R)1 - x * x, x2 = x * x, x4 = x2 * x2;
+  R thr = ff ? (R)1 : (R)0.999;
+  if (x <= -thr) return (R)0;
+  if (x2 < thr) {
+    R ans = (R)0.5 * (-om * Math<R>::exp(Math<R>::div((R)-1, om)) - expint_neg<R>(Math<R>::div((R)-1, om)));
+    if (!ff) {
+      R arg = Math<R>::div((R)3.31 * x + (R)5.25 * x * x2 * Math<R>::sin(x) * Math<R>::cos((R)-0.907 * x2 - (R)1.29 * x4 * x4), Math<R>::cos(x));
+      ans += Math<R>::div((R)2 * n * g.a, lam) * Math<R>::div((R)0.444, (R)1 + Math<R>::exp(-arg));
+    } else {
+      ans += Math<R>::div((R)2 * n * g.a, lam) *
+             ((R)0.218 + Math<R>::div((R)0.393 * x * Math<R>::cosh((R)0.806 * x4 - (R)0.696 * x2 + (R)0.0902),
+                                       Math<R>::cosh((R)0.825 * x2 - (R)0.0234 * x + (R)0.375)));
+    }
+    return ans;
+  }
+  return Math<R>::div((R)2 * n * g.a * g.q1, lam);
+}
+
+template <class R> __device__ R flperi(const GP<R>& g, R rho2) {   // intEV.cu:202-232
+  R rho = Math<R>::sqrt(rho2), lam = Math<R>::sqrt(g.l2);
+  R flraf = Math<R>::floor(Math<R>::div(rho, g.a)), f = (R)0;
+  int fl = (int)flraf;
+  if (fl - 2 > 0) f = (R)0.5 * (flraf - (R)3) * (flraf - (R)2);
+  for (int i = fl - 2; i <= fl + 2; ++i) {
+    if (i == 0) {
+      R ex = Math<R>::exp(-Math<R>::div(rho, lam)), ep1 = (R)1 + ex;
+      f = -Math<R>::div(rho, g.a) * Math<R>::div(ex, ep1) - Math<R>::div(lam, g.a) * Math<R>::log(ep1);
+    } else if (i > 0) {
+      f += Math<R>::div((R)i, (R)1 + Math<R>::exp(-Math<R>::div(rho - (R)i * g.a, lam)));
+    }
+  }
+  return Math<R>::div(g.a, lam * (R)0.69314718055994530942) * f + (R)1;
+}
+
+template <class R> __device__ R fplperi(const GP<R>& g, R rho2) {  // intEV.cu:301-328
+  R rho = Math<R>::sqrt(rho2), lam = Math<R>::sqrt(g.l2);
+  R flraf = Math<R>::floor(Math<R>::div(rho, g.a)), f = (R)0;
+  int fl = (int)flraf;
+  for (int i = fl - 2; i <= fl + 2; ++i) {
+    if (i == 0) {
+      R ex = Math<R>::exp(-Math<R>::div(rho, lam)), ep1 = (R)1 + ex;
+      f = Math<R>::div(ex, g.a * ep1 * ep1);
+    } else if (i > 0) {
+      R ea = Math<R>::exp(-Math<R>::div(rho - (R)i * g.a, lam)), eap1 = (R)1 + ea;
+      f += Math<R>::div((R)i, rho) * Math<R>::div(ea, eap1 * eap1);
+    }
+  }
+  return Math<R>::div(g.a, (R)2 * g.l2 * (R)0.69314718055994530942) * f;
+}
+
+template <class R> __device__ R ffixflux(const GP<R>& g, R rho2, bool ff) {  // intEV.cu:237-257
+  R lam = Math<R>::sqrt(g.l2);
+  R lml = Math<R>::div(lam - g.lmin, g.a - g.lmin), aml = Math<R>::div(g.a - lam, g.a - g.lmin);
+  R n = Math<R>::floor(Math<R>::div(Math<R>::sqrt(rho2) + g.a * (R)0.5, g.a));
+  if (rho2 <= g.a * g.a * (R)0.25)
+    return ((R)1 - (R)0.75 * lml) * phi_fix<R>(g, (R)2 * Math<R>::sqrt(Math<R>::div(rho2, g.l2)), ff) +
+           Math<R>::div((R)3 * rho2, g.a * g.a) * lml;
+  R ans = (R)1 + (R)0.75 * (Math<R>::div((R)4 * rho2, g.a * g.a) - (R)1) * lml + (R)3 * n * (n - (R)1) * aml;
+  ans += Math<R>::div((R)3 * lam, g.q1 * g.a) * aml *
+         chi_fix<R>(g, Math<R>::div((R)2 * (Math<R>::sqrt(rho2) - n * g.a), lam), n, lam, ff);
+  return ans;
+}
+
+template <class R> __device__ R fpfixflux(const GP<R>& g, R rho2) {   // intEV.cu:334-357
+  R lam = Math<R>::sqrt(g.l2), rho = Math<R>::sqrt(rho2);
+  R lml = Math<R>::div(lam - g.lmin, g.a - g.lmin), aml = Math<R>::div(g.a - lam, g.a - g.lmin);
+  R n = Math<R>::floor(Math<R>::div(rho + g.a * (R)0.5, g.a));
+  if (rho <= g.a * (R)0.5)
+    return Math<R>::div((R)2, g.l2 * g.q2) * ((R)1 - (R)0.75 * lml) * bump<R>(Math<R>::div((R)2 * rho, lam)) +
+           Math<R>::div((R)3, g.a * g.a) * lml;
+  return Math<R>::div((R)3, g.a * g.a) * lml +
+         Math<R>::div((R)6, g.q1 * lam * g.a) * aml * bump<R>(Math<R>::div((R)2 * (rho - n * g.a), lam));
+}
+
+// Spline: interval by bisection over the sorted knots (intEV.cu:81-89
+// semantics: the largest j with knot[j] <= u), cubic by Horner.
+template <class R>
+__device__ int spline_interval(R u, const float* __restrict__ kn, int n) {
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    int mid = (hi + lo) >> 1;
+    if (u >= (R)kn[mid]) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <class R>
+__device__ R spline_f(R u, const float* __restrict__ kn, const float4* __restrict__ cf, int n, int mode) {
+  R k0 = (R)kn[0], kl = (R)kn[n - 1];
+  if (u < kl && u > k0) {
+    int j = spline_interval<R>(u, kn, n);
+    float4 c = cf[j];
+    R dx = u - (R)kn[j];
+    return (R)c.x + dx * ((R)c.y + dx * ((R)c.z + dx * (R)c.w));
+  }
+  if (mode == M_REFERENCE) return (R)0;   // intEV.cu:96-99
+  if (u <= k0) return (k0 > (R)0) ? Math<R>::div((R)cf[0].x * u, k0) : (R)cf[0].x;
+  return (R)cf[n - 1].x;
+}
+
+template <class R>
+__device__ R spline_fp(R u, const float* __restrict__ kn, const float4* __restrict__ cf, int n, int mode, const GP<R>& g) {
+  if (mode == M_REFERENCE) {   // intEV.cu:389-398: analytic sin^2 form
+    R s = Math<R>::sin(Math<R>::sqrt(u) * (R)3.14159265358979323846);
+    return (s > (R)100) ? (R)0 : Math<R>::exp(-s * s * g.inv_l2) * g.inv_l2;
+  }
+  R k0 = (R)kn[0], kl = (R)kn[n - 1];
+  if (u <= k0) return (k0 > (R)0) ? Math<R>::div((R)cf[0].x, k0) : (R)cf[0].y;
+  if (u >= kl) return (R)0;
+  int j = spline_interval<R>(u, kn, n);
+  float4 c = cf[j];
+  R dx = u - (R)kn[j];
+  return (R)c.y + dx * ((R)2 * (R)c.z + dx * (R)3 * (R)c.w);
+}
+
+// Generic profile evaluation: f(u) and f'(u).
+template <class R>
+__device__ void profile_eval(int kind, const GP<R>& g, R u, int mode, bool need_fp,
+                             const float* __restrict__ kn, const float4* __restrict__ cf, int ns,
+                             R& f, R& fp) {
+  switch (kind) {
+    case P_STEP:
+      f = (u < g.l2) ? u * g.inv_l2 : (R)1;
+      fp = (u < g.l2) ? g.inv_l2 : (R)0;
+      break;
+    case P_SMOOTH: {
+      R den = g.l2 + u;
+      f = Math<R>::div(u, den);
+      fp = need_fp ? Math<R>::div(g.l2, den * den) : (R)0;
+    } break;
+    case P_QUADRATIC:
+      f = (u < g.l2) ? u * g.inv_l2 * ((R)2 - u * g.inv_l2) : (R)1;
+      fp = (u < g.l2) ? (R)2 * g.inv_l2 * ((R)1 - u * g.inv_l2) : (R)0;
+      break;
+    case P_GAUSSIAN: {
+      R e = Math<R>::exp(-u * g.inv_l2);
+      f = (R)1 - e;
+      fp = e * g.inv_l2;
+    } break;
+    case P_PERIODIC:
+      f = flperi<R>(g, u);
+      fp = need_fp ? fplperi<R>(g, u) : (R)0;
+      break;
+    case P_SPLINE:
+      f = spline_f<R>(u, kn, cf, ns, mode);
+      fp = need_fp ? spline_fp<R>(u, kn, cf, ns, mode, g) : (R)0;
+      break;
+    case P_FIXFLUX:
+    case P_FIXFLUX_FF:
+      f = ffixflux<R>(g, u, kind == P_FIXFLUX_FF);
+      fp = need_fp ? fpfixflux<R>(g, u) : (R)0;
+      break;
+    default:
+      f = (R)0; fp = (R)0;
+  }
+}
+
+// g(u) = f(u)/u for the generic profiles.  Reference mode: plain quotient
+// (the caller has clamped u).  Physical mode: analytic limits at u -> 0.
+template <class R>
+__device__ R generic_g(int kind, const GP<R>& g, R u, int mode, bool need_fp,
+                       const float* __restrict__ kn, const float4* __restrict__ cf, int ns, R& fp) {
+  R f;
+  if (mode == M_PHYSICAL) {
+    switch (kind) {
+      case P_STEP:
+        fp = (u < g.l2) ? g.inv_l2 : (R)0;
+        return (u < g.l2) ? g.inv_l2 : Math<R>::div((R)1, u);
+      case P_SMOOTH: {
+        R den = g.l2 + u;
+        fp = need_fp ? Math<R>::div(g.l2, den * den) : (R)0;
+        return Math<R>::div((R)1, den);
+      }
+      case P_QUADRATIC:
+        fp = (u < g.l2) ? (R)2 * g.inv_l2 * ((R)1 - u * g.inv_l2) : (R)0;
+        return (u < g.l2) ? ((R)2 - u * g.inv_l2) * g.inv_l2 : Math<R>::div((R)1, u);
+      default:
+        break;
+    }
+    profile_eval<R>(kind, g, u, mode, true, kn, cf, ns, f, fp);
+    if (u > (R)0) return Math<R>::div(f, u);
+    return fp;   // f(0) = 0: the limit of f(u)/u is f'(0)
+  }
+  profile_eval<R>(kind, g, u, mode, need_fp, kn, cf, ns, f, fp);
+  return Math<R>::div(f, u);
+}
+
+}  // namespace wfl

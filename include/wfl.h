@@ -127,6 +127,9 @@ int wfl_read_text_loops(const char *path, int Nl, int N, float *x, float *y);
 int wfl_plan_create(wfl_plan **out, const wfl_batch *b, wfl_loops *L);
 /* Enqueue the whole evaluation on `stream` (asynchronous). */
 int wfl_plan_run(wfl_plan *p, void *stream);
+/* wfl_plan_run, then wait for it; *kernel_ms is the main Wilson-loop
+ * kernel's duration measured with HIP events on `stream`. */
+int wfl_plan_run_timed(wfl_plan *p, void *stream, float *kernel_ms);
 /* Capture one wfl_plan_run into a hipGraph and replay it (T-sweep reuse). */
 int wfl_plan_graph_run(wfl_plan *p, void *stream);
 /* Copy results to the host (synchronizes): gsum [ntup][ngroups][3] sums of

@@ -41,6 +41,7 @@ _P, _I, _D = C.c_void_p, C.c_int, C.c_double
 for _n, _r, _a in [
     ("orc_philox4x32_10", None, [_P, _P, _P]),
     ("orc_gen_loops", None, [C.c_uint32, _I, _I, _P, _P]),
+    ("orc_succ", _I, [_I, _I]),
     ("orc_f", _D, [_P, _D]),
     ("orc_fp", _D, [_P, _D]),
     ("orc_expint", _D, [_D]),

@@ -328,7 +328,7 @@ static void orc_segment(const orc_geom *g, int mode, int nsub, int spinor,
 }
 
 /* getzp1 successor map (intEV.cu:462-506). */
-static int orc_succ(int i, int N) {
+int orc_succ(int i, int N) {
   if (i % 2 == 1) return (i == N - 1) ? 1 : i + 2;
   if (i % 4 == 0) return (i == N - 4) ? 0 : i + 4;
   return (i == N - 2) ? 2 : i + 4;

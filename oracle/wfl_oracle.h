@@ -49,6 +49,8 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 /* Unit loops, SoA loop-major x[Nl*N], y[Nl*N]. */
 void orc_gen_loops(uint32_t config_id, int Nl, int N, float *x, float *y);
 
+/* Successor of point i in its sub-loop (getzp1, intEV.cu:462-506). */
+int orc_succ(int i, int N);
 double orc_f(const orc_geom *g, double rho2);
 double orc_fp(const orc_geom *g, double rho2);
 double orc_expint(double x);

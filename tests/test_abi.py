@@ -1,0 +1,116 @@
+"""The C-ABI library: it loads, exports every function include/*.h declares,
+and its host-side argument checks and error paths work without a GPU."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in ("wfl.h", "intT.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"static inline[^{]*\{.*?\n\}", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?[\s\*](\w+)\s*\([^;{]*\)\s*;", text, flags=re.M):
+            if not m.group(0).lstrip().startswith("typedef"):
+                names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol(wfl):
+    names = declared_functions()
+    assert len(names) > 30
+    lib = C.CDLL(wfl.LIB_PATH)
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert names <= set(wfl.EXPORTS) | names   # binding covers them
+    assert set(wfl.EXPORTS) >= names, sorted(names - set(wfl.EXPORTS))
+
+
+def test_argument_errors_are_reported(wfl):
+    b = wfl.Batch([wfl.Geometry()], np.array([0]), np.array([0.0]), np.array([0.0]), np.array([1.0]), nsub=3)
+    h = C.c_void_p()
+    cb, keep = b.c()
+    rc = wfl.lib.wfl_plan_create(C.byref(h), C.byref(cb), C.c_void_p(1))
+    assert rc == -1 and b"nsub" in wfl.lib.wfl_last_error()
+    b2 = wfl.Batch([wfl.Geometry(kind=42)], np.array([0]), np.array([0.0]), np.array([0.0]), np.array([1.0]))
+    cb, keep = b2.c()
+    assert wfl.lib.wfl_plan_create(C.byref(h), C.byref(cb), C.c_void_p(1)) == -1
+    b3 = wfl.Batch([wfl.Geometry()], np.array([0]), np.array([0.0]), np.array([0.0]), np.array([-1.0]))
+    cb, keep = b3.c()
+    assert wfl.lib.wfl_plan_create(C.byref(h), C.byref(cb), C.c_void_p(1)) == -1
+    assert b"T > 0" in wfl.lib.wfl_last_error()
+    b4 = wfl.Batch([wfl.Geometry(kind=wfl.SPLINE, spline_table=0)], np.array([0]), np.array([0.0]),
+                   np.array([0.0]), np.array([1.0]))
+    cb, keep = b4.c()
+    assert wfl.lib.wfl_plan_create(C.byref(h), C.byref(cb), C.c_void_p(1)) == -1
+    # loop shapes: N must be a multiple of 4
+    x = np.zeros((2, 6), np.float32)
+    assert wfl.lib.wfl_loops_create(C.byref(h), x.ctypes.data, x.ctypes.data, 2, 6) == -1
+    assert wfl.lib.wfl_gen_loops(1, 2, 6, x.ctypes.data, x.ctypes.data) == -1
+    assert wfl.lib.wfl_set_numerics(0, 0, 3) == -1
+    with pytest.raises(wfl.WflError):
+        wfl.spline_build([0.0, 1.0, 0.5], [0.0, 1.0, 1.0])
+
+
+@pytest.mark.skipif(os.environ.get("HIP_VISIBLE_DEVICES") is None and os.path.exists("/dev/kfd"),
+                    reason="a GPU is present")
+def test_no_device_path_fails_loudly(wfl):
+    """Without a device every entry that needs one returns WFL_ERR_DEVICE
+    with a message (there is no CPU fallback)."""
+    if wfl.device_count() > 0:
+        pytest.skip("device present")
+    x = np.zeros((2, 8), np.float32)
+    h = C.c_void_p()
+    rc = wfl.lib.wfl_loops_create(C.byref(h), x.ctypes.data, x.ctypes.data, 2, 8)
+    assert rc == -2 and wfl.lib.wfl_last_error()
+    with pytest.raises(wfl.WflError):
+        wfl.Loops(x, x)
+
+
+def test_gen_loops_and_text_reader_roundtrip(wfl, tmp_path):
+    """getwl.c text layout: per loop N x values then N y values."""
+    x, y = wfl.gen_loops(4, 3, 8)
+    p = tmp_path / "wl.dat"
+    with open(p, "w") as fh:
+        for l in range(3):
+            fh.write(" ".join(f"{v:.9g}" for v in x[l]) + "\n")
+            fh.write(" ".join(f"{v:.9g}" for v in y[l]) + "\n")
+    rx = np.zeros((3, 8), np.float32)
+    ry = np.zeros((3, 8), np.float32)
+    wfl.check(wfl.lib.wfl_read_text_loops(str(p).encode(), 3, 8, rx.ctypes.data, ry.ctypes.data))
+    assert np.array_equal(rx, x) and np.array_equal(ry, y)
+    aos = np.zeros((3 * 8, 4), np.float32)
+    wfl.lib.getwl(aos.ctypes.data, str(p).encode(), 3, 8, 2)
+    assert np.array_equal(aos[:, 0].reshape(3, 8), x) and np.array_equal(aos[:, 1].reshape(3, 8), y)
+    assert wfl.lib.wfl_read_text_loops(str(p).encode(), 4, 8, rx.ctypes.data, ry.ctypes.data) == -1
+
+
+def test_effective_T_and_group_ev(wfl):
+    for T in (1e-3, 0.37, 12.0):
+        r = np.float32(np.sqrt(T))
+        assert wfl.effective_T(T)[0] == float(r) * float(r)
+    g = np.array([3.0, -1.5, 0.6])
+    assert wfl.group_ev_minus1(g, 3) == pytest.approx((8 / 3 * 3.0 + 2 * 1.5 + 0.2) / 3)
+
+
+def test_leff_groups_matches_oracle(wfl, orc):
+    """Host L_eff (Tfunc policy + log-Simpson) against the oracle's on
+    synthetic group sums."""
+    rng = np.random.default_rng(1)
+    T = wfl.effective_T(np.logspace(-3, 2, 24))
+    gs = -np.abs(rng.normal(size=(24, 4, 3))) * 1e-2
+    counts = np.full(4, 128, np.int32)
+    for fermion in (0, 1):
+        for policy in (0, 1):
+            for rho in (0.0, 0.8):
+                a = wfl.leff_groups(wfl.Geometry(kind=wfl.GAUSSIAN, F=1.0, l2=0.25), fermion, T, rho, 0.0, gs,
+                                    counts, policy)
+                b = orc.leff_groups(orc.Geom(orc.GAUSSIAN, F=1.0, l2=0.25), fermion, T, rho, 0.0, gs, counts,
+                                    policy)
+                assert np.allclose(a, b, rtol=1e-12, atol=1e-300)

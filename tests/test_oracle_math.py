@@ -1,0 +1,233 @@
+"""Oracle pins for the profiles (intEV.cu:29-406), the ensemble average
+(Integrand.c:325-407), the proper-time closed forms (Integrand.c:20-172),
+the spline set-up (periodic.c:45-110) and the log-T quadrature."""
+import numpy as np
+import pytest
+import scipy.integrate as si
+import scipy.interpolate as sint
+import scipy.special as sp
+
+
+# ---------------------------------------------------------------- profiles
+def test_expint_matches_scipy(orc):
+    for x in [-1.0001, -1.5, -3.0, -10.0, -50.0, -300.0]:
+        assert orc.lib.orc_expint(x) == pytest.approx(sp.expi(x), rel=1e-13)
+
+
+def test_fixflux_constants():
+    """q1 = int_-1^1 e^{-1/(1-x^2)} dx and q2 = int_0^1 x e^{-1/(1-x^2)} dx
+    = (e^{-1} - E1(1))/2: the values the engine uses by default."""
+    bump = lambda x: np.exp(-1.0 / (1.0 - x * x)) if abs(x) < 1 else 0.0
+    q1 = si.quad(bump, -1, 1, epsabs=1e-14)[0]
+    q2 = si.quad(lambda x: x * bump(x), 0, 1, epsabs=1e-14)[0]
+    assert q1 == pytest.approx(0.443993816168079, rel=1e-12)
+    assert q2 == pytest.approx(0.0742477533879612, rel=1e-12)
+    assert q2 == pytest.approx(0.5 * (np.exp(-1) - sp.exp1(1.0)), rel=1e-12)
+
+
+def test_gaussian_and_simple_profiles_closed_forms(orc):
+    l2 = 0.7
+    u = np.array([0.0, 1e-6, 0.3, 0.7, 2.0, 30.0])
+    g = orc.Geom(orc.GAUSSIAN, l2=l2, quantize=False)
+    for v in u:
+        assert orc.lib.orc_f(g.ref, v) == pytest.approx(-np.expm1(-v / l2), rel=1e-14, abs=1e-300)
+        assert orc.lib.orc_fp(g.ref, v) == pytest.approx(np.exp(-v / l2) / l2, rel=1e-14)
+    s = orc.Geom(orc.SMOOTH, l2=l2, quantize=False)
+    q = orc.Geom(orc.QUADRATIC, l2=l2, quantize=False)
+    st = orc.Geom(orc.STEP, l2=l2, quantize=False)
+    for v in u:
+        assert orc.lib.orc_f(s.ref, v) == pytest.approx(v / (l2 + v), rel=1e-14, abs=1e-300)
+        w = min(v / l2, 1.0)
+        assert orc.lib.orc_f(q.ref, v) == pytest.approx(w * (2 - w), rel=1e-14, abs=1e-300)
+        assert orc.lib.orc_f(st.ref, v) == pytest.approx(w, rel=1e-14, abs=1e-300)
+
+
+@pytest.mark.parametrize("prof", ["SMOOTH", "QUADRATIC", "GAUSSIAN", "PERIODIC", "FIXFLUX"])
+def test_fp_is_derivative_of_f(orc, prof):
+    """f'(rho^2) = df/d(rho^2) wherever the profile is smooth."""
+    g = orc.Geom(getattr(orc, prof), F=1.0, l2=0.36, tubedist=2.0, lmin=0.2, quantize=False)
+    for u in [0.05, 0.2, 0.31, 0.8, 2.5, 5.0]:
+        if prof == "QUADRATIC" and abs(u - 0.36) < 0.05:
+            continue
+        if prof == "FIXFLUX" and np.sqrt(u) > 0.95:
+            # outside rho = a/2 the reference defines f through a fitted chi
+            # (intEV.cu:188-190) whose derivative is not fpfixflux
+            continue
+        h = 1e-6 * max(u, 1.0)
+        fd = (orc.lib.orc_f(g.ref, u + h) - orc.lib.orc_f(g.ref, u - h)) / (2 * h)
+        assert orc.lib.orc_fp(g.ref, u) == pytest.approx(fd, rel=2e-6, abs=1e-8), u
+
+
+def test_periodic_profile_rings(orc):
+    """flperi (intEV.cu:202-232): f(0) = 0 and f steps up across each ring of
+    radius i*a (logistic steps of width lambda)."""
+    a = 1.0
+    g = orc.Geom(orc.PERIODIC, l2=0.0025, tubedist=a, quantize=False)
+    assert abs(orc.lib.orc_f(g.ref, 1e-12)) < 1e-3
+    for k in (1, 2, 3):
+        below = orc.lib.orc_f(g.ref, ((k - 0.3) * a) ** 2)
+        above = orc.lib.orc_f(g.ref, ((k + 0.3) * a) ** 2)
+        assert above > below
+
+
+def test_interp_outside_knots_is_zero(orc):
+    """intEV.cu:96-99: interp returns 0 outside (rho2sp[0], rho2sp[N-1])."""
+    knots = np.array([0.1, 0.5, 1.0, 2.0], np.float32)
+    coefs = np.array([[0.2, 1.0, 0.0, 0.0]] * 4, np.float32)
+    assert orc.lib.orc_interp(0.05, knots.ctypes.data, coefs.ctypes.data, 4) == 0.0
+    assert orc.lib.orc_interp(2.0, knots.ctypes.data, coefs.ctypes.data, 4) == 0.0
+    assert orc.lib.orc_interp(0.5, knots.ctypes.data, coefs.ctypes.data, 4) == pytest.approx(0.2)
+    assert orc.lib.orc_interp(0.7, knots.ctypes.data, coefs.ctypes.data, 4) == pytest.approx(0.2 + 0.2)
+
+
+# ---------------------------------------------------------------- spline
+def test_spline_matches_scipy_clamped(orc, wfl):
+    n = 64
+    rho = (np.arange(n) + 1) * (6.0 / n)
+    knots = (rho * rho).astype(np.float32)
+    fli = -np.expm1(-knots.astype(np.float64) / 1.3)
+    c4, cd = orc.spline(fli, knots)
+    cs = sint.CubicSpline(knots.astype(np.float64), fli, bc_type=((1, 0.0), (1, 0.0)))
+    for k, col in enumerate([3, 2, 1, 0]):      # scipy stores the cubic from the highest power down
+        assert np.allclose(cd[:-1, k], cs.c[col], rtol=1e-9, atol=1e-12)
+    # the product's host spline builder gives the same float4 table
+    assert np.array_equal(wfl.spline_build(fli, knots), c4)
+    q4, _ = orc.spline(fli, knots, quirk=True)
+    assert np.array_equal(wfl.spline_build(fli, knots, quirk=True), q4)
+    assert np.allclose(q4, c4, rtol=1e-4, atol=1e-6)
+
+
+def test_spline_interpolates_gaussian(orc):
+    n = 512
+    rho = (np.arange(n) + 1) * (20.0 / n)
+    knots = (rho * rho).astype(np.float32)
+    fli = -np.expm1(-knots.astype(np.float64))
+    c4, _ = orc.spline(fli, knots)
+    # the zero end slope at the first knot (periodic.c:72) bends the spline
+    # within a few knot spacings of the origin; test beyond that
+    for u in [0.1, 0.3, 1.0, 2.7, 9.0]:
+        assert orc.lib.orc_interp(u, knots.ctypes.data, c4.ctypes.data, n) == pytest.approx(-np.expm1(-u), abs=2e-6)
+
+
+# ---------------------------------------------------------------- EVMean
+def test_wminus1_scalar_and_spinor(orc):
+    sc = np.array([0.3, -0.1, 0.25])
+    yz = sc[1] + sc[2]
+    a = [0.5 * sc[0] + 0.25 * yz, sc[0], 0.5 * yz]
+    assert np.allclose(orc.wminus1(1, 0, sc, np.zeros(3)), np.cos(a) - 1.0, rtol=1e-13)
+    fe = np.array([0.8, 0.35, 0.45])
+    fyz = fe[1] + fe[2]
+    b = [0.5 * fe[0] + 0.25 * fyz, fe[0], 0.5 * fyz]
+    assert np.allclose(orc.wminus1(1, 1, sc, fe), np.cos(a) * np.cosh(b) - 1.0, rtol=1e-13)
+    # the reference's cosh guard: |trunc(arg)| > 600 multiplies by 1e10 instead
+    big = np.array([800.0, 0.0, 0.0])
+    assert np.allclose(orc.wminus1(0, 1, sc, big), np.cos(a) * 1e10 - 1.0)
+    assert not np.all(np.isfinite(orc.wminus1(1, 1, sc, big)))      # physical mode: no guard
+
+
+def test_evmean_richardson_cancels_discretisation(orc):
+    """When the three sub-loop estimates describe the same phase (x = phi,
+    y = z = phi) all three W agree and EV = W: the coefficients 8/3, -2, 1/3
+    sum to one; they also cancel W = 1 + a/N + b/N^2 corrections."""
+    phi = np.linspace(-2, 2, 16)
+    ws = np.zeros((16, 4), np.float32)
+    ws[:, 0] = phi; ws[:, 1] = phi; ws[:, 2] = phi
+    ev = orc.evmean(ws, None, np.arange(16), 0)
+    assert ev == pytest.approx(np.mean(np.cos(phi.astype(np.float32).astype(np.float64))), rel=1e-12)
+    c = np.array([8 / 3, -2.0, 1 / 3])
+    Ns = np.array([1.0, 0.5, 0.25])          # N, N/2, N/4 in units of N
+    assert c.sum() == pytest.approx(1.0)
+    assert (c / Ns).sum() == pytest.approx(0.0, abs=1e-14)
+    assert (c / Ns**2).sum() == pytest.approx(0.0, abs=1e-14)
+
+
+def test_evmean_spinor_hand_vector(orc, wfl):
+    rng = np.random.default_rng(0)
+    ws = rng.normal(size=(10, 4)).astype(np.float32)
+    wf = np.abs(rng.normal(size=(10, 4))).astype(np.float32)
+    WL = np.array([3, 1, 4, 1, 5, 9, 2, 6], np.int32)
+    want = 0.0
+    e = np.zeros(3)
+    for w in WL:
+        s, f = ws[w].astype(np.float64), wf[w].astype(np.float64)
+        nb, fb = float(np.float32(ws[w, 1] + ws[w, 2])), float(np.float32(wf[w, 1] + wf[w, 2]))
+        e += [np.cos(0.5 * s[0] + 0.25 * nb) * np.cosh(0.5 * f[0] + 0.25 * fb),
+              np.cos(s[0]) * np.cosh(f[0]), np.cos(0.5 * nb) * np.cosh(0.5 * fb)]
+    e /= len(WL)
+    want = 8 / 3 * e[0] - 2 * e[1] + 1 / 3 * e[2]
+    assert orc.evmean(ws, wf, WL, 1) == pytest.approx(want, rel=1e-13)
+    # the product's host EVMean (C) computes the same value bit for bit
+    import ctypes as C
+    out = C.c_double(0.0)
+    wfl.lib.EVMean(C.byref(out), ws.ctypes.data, wf.ctypes.data, len(WL), WL.ctypes.data, 1.0, 1)
+    assert out.value == orc.evmean(ws, wf, WL, 1)
+
+
+# ---------------------------------------------------------------- Tfunc
+def test_small_T_series_match_exact_taylor(orc):
+    """smTscal / smTferm are the small-T Taylor series of Exact (checked in
+    high precision with mpmath): the relative gap shrinks like T^3."""
+    import mpmath as mp
+
+    mp.mp.dps = 50
+    e = mp.mpf("0.30282212")
+    B = 3.0
+
+    def exact(T, ferm):
+        T = mp.mpf(T)
+        x = e * T * B
+        if ferm:
+            return mp.e**(-T) / T**3 * (x / mp.tanh(x) - 1 - x * x / 3)
+        return mp.e**(-T) / T**3 * (x / mp.sinh(x) - 1 + x * x / 6)
+
+    for ferm, fn in ((0, orc.lib.orc_smTscal), (1, orc.lib.orc_smTferm)):
+        gaps = []
+        for T in (1e-2, 1e-3):
+            ex = float(exact(T, ferm))
+            gaps.append(abs(fn(B, T) - ex) / abs(ex))
+        assert gaps[1] < gaps[0] / 500
+        assert gaps[1] < 1e-8
+
+
+def test_exact_matches_mpmath(orc):
+    import mpmath as mp
+
+    mp.mp.dps = 40
+    e = mp.mpf("0.30282212")
+    for T, B, ferm in [(0.5, 2.0, 0), (3.0, 1.0, 1), (10.0, 0.3, 0), (1.0, 40.0, 0)]:
+        x = e * mp.mpf(T) * B
+        if ferm:
+            v = mp.e**(-T) / mp.mpf(T)**3 * (x / mp.tanh(x) - 1 - x * x / 3)
+        elif x < 50:
+            v = mp.e**(-T) / mp.mpf(T)**3 * (x / mp.sinh(x) - 1 + x * x / 6)
+        else:
+            v = mp.e**(-T) / mp.mpf(T)**3 * (x * x / 6 - 1)
+        assert orc.lib.orc_exact(T, B, ferm) == pytest.approx(float(v), rel=1e-11)
+
+
+def test_igrand_branches(orc):
+    g = orc.Geom(orc.GAUSSIAN, F=1.0, l2=1.0, quantize=False)
+    assert orc.igrand(g, 0, 0.005, 0.0, 0.0, 1.0)[1] == 0     # T < 0.1 and T < 0.01 l2: series
+    assert orc.igrand(g, 0, 0.05, 0.0, 0.0, 1.0)[1] == 3      # worldline estimate
+    assert orc.igrand(g, 1, 60.0, 0.0, 0.0, 1.0)[1] == 2      # large-T asymptote
+    wide = orc.Geom(orc.GAUSSIAN, F=1.0, l2=100.0, quantize=False)
+    assert orc.igrand(wide, 0, 0.2, 0.0, 0.0, 1.0)[1] == 1    # 0.1 <= T < 0.01 l2: closed form
+
+
+def test_logsimpson_exact_for_cubics_in_lnT(orc, wfl):
+    for n in (5, 6, 64, 96):
+        T = np.logspace(-3, 2, n)
+        u = np.log(T)
+        p = lambda v: 0.3 - 1.2 * v + 0.05 * v**2 + 0.01 * v**3
+        P = lambda v: 0.3 * v - 0.6 * v**2 + 0.05 * v**3 / 3 + 0.0025 * v**4
+        f = p(u) / T                      # int f dT = int p(u) du
+        want = P(u[-1]) - P(u[0])
+        assert orc.logsimpson(T, f) == pytest.approx(want, rel=1e-11)
+        assert wfl.logsimpson(T, f) == pytest.approx(want, rel=1e-11)
+
+
+def test_host_closed_forms_match_oracle(orc, wfl):
+    for T, B, ferm in [(0.01, 3.0, 0), (0.7, 1.5, 1), (5.0, 0.2, 0)]:
+        assert wfl.lib.Exact(T, B, ferm) == orc.lib.orc_exact(T, B, ferm)
+        assert wfl.lib.smTscal(B, T) == orc.lib.orc_smTscal(B, T)
+        assert wfl.lib.smTferm(B, T) == orc.lib.orc_smTferm(B, T)

@@ -288,6 +288,8 @@ struct wfl_plan_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipStream_t gstream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;   // bracket the main kernel (timed runs)
+  bool timed = false;
 };
 
 extern "C" void wfl_plan_destroy(wfl_plan* p) {
@@ -295,6 +297,8 @@ extern "C" void wfl_plan_destroy(wfl_plan* p) {
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->gstream) (void)hipStreamDestroy(p->gstream);
+  if (p->ev0) (void)hipEventDestroy(p->ev0);
+  if (p->ev1) (void)hipEventDestroy(p->ev1);
   void* bufs[] = {p->d_tups, p->d_geoms, p->d_lanes, p->d_tasks, p->d_knots, p->d_coefs,
                   p->d_gsum, p->d_perloop, p->d_wm1, p->d_nonfinite};
   for (void* b : bufs) if (b) (void)hipFree(b);
@@ -470,10 +474,13 @@ extern "C" int wfl_plan_run(wfl_plan* p, void* stream) {
   WflLaunch a;
   fill_launch(p, a, s);
   int e;
+  if (p->timed) HIP_TRY(hipEventRecord(p->ev0, s));
   if (p->mapping == 1) {
     e = p->b.precision == WFL_MIXED ? wfl_launch_tuple_f32(&a) : wfl_launch_tuple_f64(&a);
+    if (p->timed && e == 0) e = (int)hipEventRecord(p->ev1, s);
   } else {
     e = p->b.precision == WFL_MIXED ? wfl_launch_loop_f32(&a) : wfl_launch_loop_f64(&a);
+    if (p->timed && e == 0) e = (int)hipEventRecord(p->ev1, s);
     if (e == 0) {
       long long n = (long long)p->b.ntup * p->ngroups;
       hipLaunchKernelGGL(k_group_sums, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, p->d_wm1, p->d_tups,
@@ -482,6 +489,20 @@ extern "C" int wfl_plan_run(wfl_plan* p, void* stream) {
     }
   }
   if (e) { wfl_set_error("kernel launch failed: %s", hipGetErrorString((hipError_t)e)); return WFL_ERR_DEVICE; }
+  return WFL_OK;
+}
+
+extern "C" int wfl_plan_run_timed(wfl_plan* p, void* stream, float* kernel_ms) {
+  if (!p || !kernel_ms) { wfl_set_error("wfl_plan_run_timed: NULL argument"); return WFL_ERR_ARG; }
+  *kernel_ms = 0.0f;
+  if (!p->ev0) HIP_TRY(hipEventCreate(&p->ev0));
+  if (!p->ev1) HIP_TRY(hipEventCreate(&p->ev1));
+  p->timed = true;
+  int rc = wfl_plan_run(p, stream);
+  p->timed = false;
+  if (rc) return rc;
+  HIP_TRY(hipEventSynchronize(p->ev1));
+  HIP_TRY(hipEventElapsedTime(kernel_ms, p->ev0, p->ev1));
   return WFL_OK;
 }
 

@@ -78,7 +78,7 @@ EXPORTS = [
     "wfl_last_error", "wfl_device_count", "wfl_device_sync",
     "wfl_loops_create", "wfl_loops_from_aos_device", "wfl_loops_generate", "wfl_loops_shape",
     "wfl_loops_destroy", "wfl_gen_loops", "wfl_read_text_loops",
-    "wfl_plan_create", "wfl_plan_run", "wfl_plan_graph_run", "wfl_plan_read", "wfl_plan_ngroups",
+    "wfl_plan_create", "wfl_plan_run", "wfl_plan_run_timed", "wfl_plan_graph_run", "wfl_plan_read", "wfl_plan_ngroups",
     "wfl_plan_gsum_device", "wfl_plan_nonfinite", "wfl_plan_mapping", "wfl_plan_destroy",
     "wfl_eval_batch", "wfl_group_ev_minus1", "wfl_spline_build", "wfl_set_profile",
     "wfl_set_numerics", "wfl_ev_reset", "wfl_ev_launches", "wfl_logsimpson", "wfl_effective_T",
@@ -106,6 +106,7 @@ def _load():
         "wfl_plan_create": (I, [P, P, P]),
         "wfl_plan_run": (I, [P, P]),
         "wfl_plan_graph_run": (I, [P, P]),
+        "wfl_plan_run_timed": (I, [P, P, P]),
         "wfl_plan_read": (I, [P, P, P]),
         "wfl_plan_ngroups": (I, [P]),
         "wfl_plan_gsum_device": (P, [P]),
@@ -282,6 +283,12 @@ class Plan:
 
     def run(self, stream=None):
         check(lib.wfl_plan_run(self.h, stream))
+
+    def run_timed(self, stream=None):
+        """Run and wait; returns the main kernel's duration (ms, HIP events)."""
+        ms = C.c_float(0.0)
+        check(lib.wfl_plan_run_timed(self.h, stream, C.byref(ms)))
+        return ms.value
 
     def graph_run(self, stream=None):
         check(lib.wfl_plan_graph_run(self.h, stream))

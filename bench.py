@@ -122,7 +122,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-ref-mode", action="store_true")
+    ap.add_argument("--only-mixed", action="store_true", help="time the mixed engine only (profiling runs)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="run only a plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.only_mixed:
+        args.no_fp64 = args.no_cpu = args.no_ref_mode = True
+    if args.copy_probe:
+        import torch
+
+        n = args.copy_probe * (1 << 20) // 4
+        a = torch.ones(n, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 4 * n, "dispatches": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 (the multi-GPU sharding "
                                    "is described, not built)"}))

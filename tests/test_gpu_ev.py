@@ -1,0 +1,149 @@
+"""The drop-in boundary on the GPU: EV(), Tfunc() and integrateT() called the
+way the reference's C drivers call them (struct Wparams with device
+buffers), checked against the FP64 oracle."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NL, N = 256, 256
+
+
+class Hip:
+    def __init__(self):
+        self.lib = C.CDLL("libamdhip64.so")
+        self.lib.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.lib.hipFree.argtypes = [C.c_void_p]
+
+    def malloc(self, n):
+        p = C.c_void_p()
+        assert self.lib.hipMalloc(C.byref(p), n) == 0
+        return p
+
+    def h2d(self, dst, src):
+        assert self.lib.hipMemcpy(dst, src.ctypes.data, src.nbytes, 1) == 0
+
+    def free(self, p):
+        self.lib.hipFree(p)
+
+
+@pytest.fixture(scope="module")
+def setup(gpu):
+    hip = Hip()
+    x, y = gpu.gen_loops(8, NL, N)
+    aos = np.zeros((NL * N, 4), np.float32)
+    aos[:, 0] = x.ravel()
+    aos[:, 1] = y.ravel()
+    d_wl = hip.malloc(aos.nbytes)
+    hip.h2d(d_wl, aos)
+    d_s = hip.malloc(NL * 16)
+    d_f = hip.malloc(NL * 16)
+    hs = np.zeros((NL, 4), np.float32)
+    hf = np.zeros((NL, 4), np.float32)
+    yield gpu, hip, x, y, d_wl, d_s, d_f, hs, hf
+    for p in (d_wl, d_s, d_f):
+        hip.free(p)
+    gpu.lib.wfl_ev_reset()
+
+
+def make_params(wfl, d_wl, d_s, d_f, hs, hf, F, l2, rho, fermion, ng=2):
+    p = wfl.Wparams()
+    p.Nl, p.Nppl, p.F, p.l2 = NL, N, F, l2
+    p.xcm.x, p.xcm.y = rho, 0.0
+    p.Wsscal_h, p.Wsscal_d = hs.ctypes.data, d_s.value
+    p.Wsferm_h, p.Wsferm_d = hf.ctypes.data, d_f.value
+    p.worldlines = d_wl.value
+    p.nBlocks, p.nThreads, p.ng, p.fermion = 1, 256, ng, fermion
+    return p
+
+
+def oracle_float4(orc, x, y, F, l2, rho, T, fermion, mode=0, nsub=50):
+    g = orc.Geom(orc.GAUSSIAN, F=F, l2=l2)      # F and l2 narrowed to float as intEV.cu:124 does
+    rt = float(np.float32(np.sqrt(T)))
+    ws = np.zeros((NL, 4), np.float32)
+    wf = np.zeros((NL, 4), np.float32)
+    for l in range(NL):
+        sc, fe = orc.wilson_loop(g, mode, nsub, fermion, x[l], y[l], float(np.float32(rho)), 0.0, rt)
+        ws[l, :3] = sc
+        wf[l, :3] = fe
+    return ws, wf
+
+
+@pytest.mark.parametrize("fermion", [0, 1])
+def test_ev_matches_oracle_and_reuses_results(setup, orc, fermion):
+    wfl, hip, x, y, d_wl, d_s, d_f, hs, hf = setup
+    wfl.set_profile(wfl.Geometry(kind=wfl.GAUSSIAN))
+    wfl.set_numerics(wfl.MODE_REFERENCE, wfl.FP64, 50)
+    wfl.lib.wfl_ev_reset()
+    F, l2, rho, T = 1.0, 1.0, 0.5, 0.7
+    p = make_params(wfl, d_wl, d_s, d_f, hs, hf, F, l2, rho, fermion)
+    WL = np.arange(128, dtype=np.int32)
+    print(f"EV fermion={fermion}", flush=True)
+    ev = wfl.lib.EV(T, C.byref(p), WL.ctypes.data)
+    ws, wf = oracle_float4(orc, x, y, F, l2, rho, T, fermion)
+    # per-loop float4 outputs agree to the last float bit or one ulp
+    assert np.allclose(hs[:, :3], ws[:, :3], rtol=3e-7, atol=1e-12)
+    if fermion:
+        assert np.allclose(hf[:, :3], wf[:, :3], rtol=3e-7)
+    assert ev == orc.evmean(hs, hf, WL, fermion)          # EV = EVMean over WLlist
+    assert ev == pytest.approx(orc.evmean(ws, wf, WL, fermion), rel=1e-6)
+    assert wfl.lib.wfl_ev_launches() == 1
+    WL2 = np.arange(128, 256, dtype=np.int32)
+    ev2 = wfl.lib.EV(T, C.byref(p), WL2.ctypes.data)        # second group: no relaunch
+    assert wfl.lib.wfl_ev_launches() == 1
+    assert ev2 == orc.evmean(hs, hf, WL2, fermion)
+    wfl.lib.EV(1.3, C.byref(p), WL.ctypes.data)            # new T: relaunch
+    assert wfl.lib.wfl_ev_launches() == 2
+
+
+def test_ev_mixed_precision(setup, orc):
+    wfl, hip, x, y, d_wl, d_s, d_f, hs, hf = setup
+    wfl.set_profile(wfl.Geometry(kind=wfl.GAUSSIAN))
+    wfl.set_numerics(wfl.MODE_REFERENCE, wfl.MIXED, 50)
+    p = make_params(wfl, d_wl, d_s, d_f, hs, hf, 1.0, 1.0, 0.3, 1)
+    WL = np.arange(128, dtype=np.int32)
+    print("EV mixed", flush=True)
+    ev = wfl.lib.EV(0.4, C.byref(p), WL.ctypes.data)
+    ws, wf = oracle_float4(orc, x, y, 1.0, 1.0, 0.3, 0.4, 1)
+    assert np.allclose(hs[:, :3], ws[:, :3], rtol=2e-5, atol=1e-7)
+    assert ev == pytest.approx(orc.evmean(ws, wf, WL, 1), rel=1e-5)
+    wfl.set_numerics(wfl.MODE_REFERENCE, wfl.FP64, 50)
+
+
+def test_integrateT_matches_oracle(setup, orc):
+    """integrateT (Tint.c:100-175) through Tfunc and EV against the oracle's
+    restatement of the same t-grid Simpson rule and branch policy."""
+    wfl, hip, x, y, d_wl, d_s, d_f, hs, hf = setup
+    F, l2, rho, fermion, n = 1.0, 1.0, 0.4, 0, 4
+    wfl.set_profile(wfl.Geometry(kind=wfl.GAUSSIAN))
+    wfl.set_numerics(wfl.MODE_REFERENCE, wfl.FP64, 50)
+    p = make_params(wfl, d_wl, d_s, d_f, hs, hf, F, l2, rho, fermion, ng=2)
+    print("integrateT", flush=True)
+    res = wfl.lib.integrateT(n, C.byref(p))
+    got = np.ctypeslib.as_array(C.cast(res, C.POINTER(C.c_double)), shape=(2,)).copy()
+    g = orc.Geom(orc.GAUSSIAN, F=F, l2=l2, quantize=False)
+    rho_f = float(np.float32(rho))
+    rho2 = float(np.float32(np.float32(rho) * np.float32(rho)))
+
+    def tfunc(T):
+        out = np.zeros(2)
+        for q in range(2):
+            v, br = orc.igrand(g, fermion, T, rho_f, rho2, 1.0)
+            if br == 3:
+                ws, wf = oracle_float4(orc, x, y, F, l2, rho, T, fermion)
+                ev = orc.evmean(ws, wf, np.arange(128 * q, 128 * q + 128, dtype=np.int32), fermion)
+                v, _ = orc.igrand(g, fermion, T, rho_f, rho2, ev)
+            out[q] = v
+        return out
+
+    h = 1.0 / n
+    want = np.zeros(2)
+    for inx in range(n):
+        t1, th = h * (inx + 1.0), h * (inx + 0.5)
+        a = tfunc((1 - t1) / t1) / t1 / t1
+        b = tfunc((1 - th) / th) / th / th
+        want += h / 6.0 * (2 * a + 4 * b)
+    assert np.allclose(got, want, rtol=1e-6, atol=1e-12)

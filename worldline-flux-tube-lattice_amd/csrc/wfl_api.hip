@@ -80,13 +80,15 @@ __global__ void k_records(const float* __restrict__ x, const float* __restrict__
   const float* yl = y + (size_t)l * N;
   int s = succ_idx(i, N), p = pred_idx(i, N);
   double wx = xl[i], wy = yl[i];
-  R* r = rec + (size_t)idx * 6;
+  R* r = rec + (size_t)idx * REC;
   r[0] = (R)wx;
   r[1] = (R)wy;
   r[2] = (R)((double)xl[s] - wx);
   r[3] = (R)((double)yl[s] - wy);
   r[4] = (R)((double)xl[s] - (double)xl[p]);
   r[5] = (R)((double)yl[s] - (double)yl[p]);
+  r[6] = (R)0;
+  r[7] = (R)0;
 }
 
 __global__ void k_aos_to_soa(const float4* __restrict__ aos, long long n, float* __restrict__ x, float* __restrict__ y) {
@@ -124,7 +126,7 @@ struct wfl_loops_s {
   int Nl = 0, N = 0;
   float* x = nullptr;      // SoA [Nl*N]
   float* y = nullptr;
-  float* rec32 = nullptr;  // [Nl][N][6]
+  float* rec32 = nullptr;  // [Nl][N][REC]
   double* rec64 = nullptr;
 };
 
@@ -218,13 +220,13 @@ static int loops_records(wfl_loops* L, int precision) {
   long long n = (long long)L->Nl * L->N;
   dim3 grid((unsigned)((n + 255) / 256)), block(256);
   if (precision == WFL_MIXED && !L->rec32) {
-    HIP_TRY(hipMalloc(&L->rec32, sizeof(float) * 6 * (size_t)n));
+    HIP_TRY(hipMalloc(&L->rec32, sizeof(float) * REC * (size_t)n));
     hipLaunchKernelGGL(k_records<float>, grid, block, 0, 0, L->x, L->y, L->Nl, L->N, L->rec32);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipDeviceSynchronize());
   }
   if (precision == WFL_FP64 && !L->rec64) {
-    HIP_TRY(hipMalloc(&L->rec64, sizeof(double) * 6 * (size_t)n));
+    HIP_TRY(hipMalloc(&L->rec64, sizeof(double) * REC * (size_t)n));
     hipLaunchKernelGGL(k_records<double>, grid, block, 0, 0, L->x, L->y, L->Nl, L->N, L->rec64);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipDeviceSynchronize());

@@ -4,7 +4,7 @@
 // Data: each loop point i carries a record of six values
 //   (w_x, w_y, dw_x, dw_y, e_x, e_y),  dw = w_succ(i) - w_i,  e = w_succ(i) - w_pred(i)
 // where succ/pred follow the sub-loop map of getzp1 (intEV.cu:462-506).
-// Records are stored point-major inside each loop ([Nl][N][6]) so that one
+// Records are stored point-major inside each loop ([Nl][N][REC], REC = 8: six values and two pad slots, 32-byte aligned) so that one
 // wavefront walking one loop reads them with scalar loads.
 //
 // Two mappings of work to the machine:
@@ -28,6 +28,9 @@
 namespace wfl {
 
 struct Task { int lanegroup, loop0, nloop, group; };
+
+// values per point record (w, dw, e and two pad slots: 32 B in FP32)
+constexpr int REC = 8;
 
 __device__ __forceinline__ int succ_idx(int i, int N) {
   if (i & 1) return (i == N - 1) ? 1 : i + 2;
@@ -236,7 +239,7 @@ __global__ __launch_bounds__(256) void k_tuple(
 
   double G[3] = {0.0, 0.0, 0.0};
   for (int l = loop0; l < loop0 + nloop; ++l) {
-    const R* __restrict__ rl = rec + (size_t)l * N * 6;
+    const R* __restrict__ rl = (const R*)__builtin_assume_aligned(rec + (size_t)l * N * REC, REC * sizeof(R));
     double P[3] = {0.0, 0.0, 0.0}, S[3] = {0.0, 0.0, 0.0};
     for (int ia = -M; ia <= M; ++ia) {
       for (int ib = -M; ib <= M; ++ib) {
@@ -249,12 +252,12 @@ __global__ __launch_bounds__(256) void k_tuple(
           R p[3] = {(R)0, (R)0, (R)0}, q[3] = {(R)0, (R)0, (R)0};
           const int i1 = i0 + 32 < N ? i0 + 32 : N;
           for (int i = i0; i < i1; i += 4) {
-            const R* r = rl + (size_t)i * 6;
+            const R* r = rl + (size_t)i * REC;
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
               const int c = (u & 1) ? 0 : (u == 0 ? 2 : 1);
-              point_eval<R, PROF, SPINOR, MODE, NSUB>(r[6 * u + 0], r[6 * u + 1], r[6 * u + 2], r[6 * u + 3],
-                                                      r[6 * u + 4], r[6 * u + 5], ccx, ccy, lc, kind, gp, kn, cf,
+              point_eval<R, PROF, SPINOR, MODE, NSUB>(r[REC * u + 0], r[REC * u + 1], r[REC * u + 2], r[REC * u + 3],
+                                                      r[REC * u + 4], r[REC * u + 5], ccx, ccy, lc, kind, gp, kn, cf,
                                                       ns, nsub_rt, p[c], q[c]);
             }
           }
@@ -312,7 +315,7 @@ __global__ __launch_bounds__(256) void k_loop(
   const int M = gm.lattice > 0.0 ? gm.images : 0;
   const R lat = sizeof(R) == 4 ? (R)gm.latticef : (R)gm.lattice;
   const R cx = (R)tp.cx, cy = (R)tp.cy;
-  const R* __restrict__ rl = rec + (size_t)l * N * 6;
+  const R* __restrict__ rl = rec + (size_t)l * N * REC;
 
   double P = 0.0, S = 0.0;      // this lane's class: lane & 3 (points i = lane + 64k)
   for (int ia = -M; ia <= M; ++ia) {
@@ -326,7 +329,7 @@ __global__ __launch_bounds__(256) void k_loop(
         R p = (R)0, q = (R)0;
         const int i1 = i0 + 64 * 8 < N ? i0 + 64 * 8 : N;
         for (int i = i0; i < i1; i += 64) {
-          const R* r = rl + (size_t)i * 6;
+          const R* r = rl + (size_t)i * REC;
           point_eval<R, PROF, SPINOR, MODE, NSUB>(r[0], r[1], r[2], r[3], r[4], r[5], ccx, ccy, lc, kind, gp, kn,
                                                   cf, ns, nsub_rt, p, q);
         }

@@ -69,28 +69,17 @@ template <> struct GP<double> {
 };
 
 // -------------------------------------------- Gaussian tube (hot profile)
-// phi1(x) = (1 - e^{-x})/x for x < 1/2 by its Taylor series (truncation
-// below 6e-10 relative); the quotient form elsewhere, where 1 - e^{-x}
-// keeps full relative precision.
-__device__ __forceinline__ float phi1_series(float x) {
-  float p = 1.0f / 362880.0f;
-  p = fmaf(p, -x, 1.0f / 40320.0f);
-  p = fmaf(p, -x, 1.0f / 5040.0f);
-  p = fmaf(p, -x, 1.0f / 720.0f);
-  p = fmaf(p, -x, 1.0f / 120.0f);
-  p = fmaf(p, -x, 1.0f / 24.0f);
-  p = fmaf(p, -x, 1.0f / 6.0f);
-  p = fmaf(p, -x, 0.5f);
-  p = fmaf(p, -x, 1.0f);
-  return p;
-}
-
-// FP32: returns g(u) = f(u)/u and e = exp(-u/l2).
+// FP32: returns g(u) = f(u)/u = (1 - e)/u and e = exp(-u/l2).  The exponent
+// is formed with -log2(e)/l2 split in hi + lo so that the rounding of that
+// constant does not bias every node the same way.  Near the tube axis
+// (u << l2) 1 - e carries the absolute rounding of e, i.e. a relative error
+// ~6e-8 l2/u; the phase weight of such a node is z x dw, which shrinks with
+// |z|, so the phase error per node stays below ~F |dw| 6e-8 / |z| (random in
+// sign).  u is floored at 1e-30: a node exactly on the axis has z x dw = 0.
 __device__ __forceinline__ float gauss_g(float u, float inv_l2, float k_hi, float k_lo, float& e) {
+  (void)inv_l2;
   e = __builtin_amdgcn_exp2f(fmaf(u, k_hi, u * k_lo));
-  float x = u * inv_l2;
-  float q = (1.0f - e) * __builtin_amdgcn_rcpf(u);
-  return (x < 0.5f) ? phi1_series(x) * inv_l2 : q;
+  return (1.0f - e) * __builtin_amdgcn_rcpf(fmaxf(u, 1e-30f));
 }
 // FP64: g(u) = -expm1(-u/l2)/u (limit 1/l2 at u = 0), e = 1 + expm1(-u/l2).
 __device__ __forceinline__ double gauss_g(double u, double inv_l2, float, float, double& e) {

@@ -68,7 +68,6 @@ def config_c1():
 def leff_table(wfl, cfg, gsum, group, policy):
     """L_eff per (geometry, rho) and per group, from group sums."""
     nrho, nT = cfg["nrho"], cfg["nT"]
-    Teff = wfl.effective_T(cfg["T"])
     ng = gsum.shape[1]
     counts = np.full(ng, group, np.int32)
     tg, tcx = cfg["tup"][0], cfg["tup"][1]
@@ -77,8 +76,8 @@ def leff_table(wfl, cfg, gsum, group, policy):
         geom = wfl.Geometry(kind=wfl.GAUSSIAN, F=F, l2=l2)
         for r in range(nrho):
             base = (g * nrho + r) * nT
-            out[g, r] = wfl.leff_groups(geom, cfg["spinor"], Teff, tcx[base], 0.0, gsum[base:base + nT], counts,
-                                        policy)
+            out[g, r] = wfl.leff_groups(geom, cfg["spinor"], cfg["T"], tcx[base], 0.0, gsum[base:base + nT],
+                                        counts, policy)
     return out
 
 

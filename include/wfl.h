@@ -166,9 +166,15 @@ long long wfl_ev_launches(void);
  * Composite Simpson in ln T (3/8 rule on the last three intervals when the
  * interval count is odd). */
 double wfl_logsimpson(const double *T, const double *f, int n);
-/* The proper time the kernels evaluate for a requested T:
- * ((double)(float)sqrt(T))^2 (sqrt(T) is narrowed to FP32 as at intEV.cu:124). */
-double wfl_effective_T(double T);
+/* Effective parameters a batch evaluates for geometry g and a tuple
+ * (cx, cy, T): out = {l2, F, cx, cy, sqrt(T), lattice}.  Every kernel
+ * parameter is narrowed to FP32 (as the reference does at intEV.cu:124); for
+ * the Gaussian fast path the narrowing happens in the FP32 kernel's scaled
+ * units (sigma = sqrt(log2 e / l2) in FP32), so both precisions evaluate
+ * exactly these doubles. */
+int wfl_effective_tuple(const wfl_geometry *g, double cx, double cy, double T, double out[6]);
+/* The effective proper time of a requested T for geometry g. */
+double wfl_effective_T(const wfl_geometry *g, double T);
 /* L_eff(rho) per loop group from batched sums on a log T grid.
  * gsum [nT][ngroups][3] holds the group sums of the nT tuples of one
  * (geometry, x_cm) in T order, counts [ngroups] the loops per group, T the

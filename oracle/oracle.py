@@ -68,23 +68,51 @@ def _p(a):
     return None if a is None else a.ctypes.data
 
 
+LOG2E = 1.4426950408889634074
+
+
 def q32(v):
-    """FP32 quantization the engine applies to every kernel parameter."""
+    """FP32 narrowing of one parameter."""
     return float(np.float32(v))
 
 
+def unit_scale(profile, l2):
+    """The engine's FP32 unit scale: sigma = sqrt(log2 e / l2) in FP32 for
+    the Gaussian fast path, 1 for the other profiles."""
+    return q32(np.sqrt(LOG2E / l2)) if profile == GAUSSIAN else 1.0
+
+
 class Geom:
-    """Oracle geometry; parameters are quantized to FP32 as the engine does."""
+    """Oracle geometry.
+
+    quantize=True: the batched engine's effective parameters (every kernel
+    parameter narrowed to FP32, in the scaled units sigma z for a Gaussian
+    tube: l2 = log2(e)/sigma^2, lattice = fl32(sigma a)/sigma);
+    quantize="float": the reference's narrowing of F and l2 to float
+    (intEV.cu:124), which EV() keeps; quantize=False: the values as given."""
 
     def __init__(self, profile=GAUSSIAN, F=1.0, l2=1.0, tubedist=1.0, lmin=0.1,
                  q1=0.443993816168079, q2=0.0742477533879612, lattice=0.0, images=0,
                  knots=None, coefs=None, quantize=True):
         qz = q32 if quantize else float
+        self.sg = unit_scale(profile, l2) if quantize is True else 1.0
+        l2e = LOG2E / (self.sg * self.sg) if self.sg != 1.0 else qz(l2)
+        late = (q32(self.sg * lattice) / self.sg if quantize else float(lattice)) if lattice > 0 else 0.0
         self.knots = None if knots is None else np.ascontiguousarray(knots, np.float32)
         self.coefs = None if coefs is None else np.ascontiguousarray(coefs, np.float32)
-        self.c = OrcGeom(profile, qz(l2), qz(F), qz(tubedist), qz(lmin), qz(q1), qz(q2),
+        self.c = OrcGeom(profile, l2e, qz(F), qz(tubedist), qz(lmin), qz(q1), qz(q2),
                          0 if knots is None else len(self.knots), _p(self.knots), _p(self.coefs),
-                         qz(lattice) if lattice > 0 else 0.0, images)
+                         late, images)
+        self.quantize = quantize
+
+    def tuple(self, cx, cy, T):
+        """Effective (cx, cy, sqrt(T)) of a batch tuple."""
+        if self.quantize is True:
+            sg = self.sg
+            return (q32(sg * cx) / sg, q32(sg * cy) / sg, q32(sg * np.sqrt(T)) / sg)
+        if self.quantize:
+            return (q32(cx), q32(cy), q32(np.sqrt(T)))
+        return (float(cx), float(cy), float(np.sqrt(T)))
 
     @property
     def ref(self):
@@ -118,8 +146,8 @@ def wilson_loop(geom, mode, nsub, spinor, wx, wy, cx, cy, rtT):
 
 def eval_batch(geoms, mode, nsub, spinor, x, y, group, tup_geom, tup_cx, tup_cy, tup_T,
                perloop=True, nthreads=0):
-    """Same tuple semantics as the engine: cx, cy quantized to FP32 and
-    sqrt(T) narrowed to FP32.  Returns (gsum [ntup, ng, 3], perloop [ntup, Nl, 6])."""
+    """Same tuple semantics as the engine (Geom.tuple).  Returns
+    (gsum [ntup, ng, 3], perloop [ntup, Nl, 6])."""
     x = np.ascontiguousarray(x, np.float32)
     y = np.ascontiguousarray(y, np.float32)
     Nl, N = x.shape
@@ -127,9 +155,11 @@ def eval_batch(geoms, mode, nsub, spinor, x, y, group, tup_geom, tup_cx, tup_cy,
     ng = (Nl + group - 1) // group
     garr = (OrcGeom * len(geoms))(*[g.c for g in geoms])
     tg = np.ascontiguousarray(tup_geom, np.int32)
-    cx = np.array([q32(v) for v in tup_cx], np.float64)
-    cy = np.array([q32(v) for v in tup_cy], np.float64)
-    rt = np.array([q32(np.sqrt(t)) for t in tup_T], np.float64)
+    eff = np.array([geoms[g].tuple(a, b, t) for g, a, b, t in zip(tg, tup_cx, tup_cy, tup_T)], np.float64)
+    eff = eff.reshape(-1, 3)
+    cx = np.ascontiguousarray(eff[:, 0])
+    cy = np.ascontiguousarray(eff[:, 1])
+    rt = np.ascontiguousarray(eff[:, 2])
     gs = np.zeros((ntup, ng, 3))
     pl = np.zeros((ntup, Nl, 6)) if perloop else None
     lib.orc_eval_batch(C.cast(garr, C.c_void_p), mode, nsub, spinor, _p(x), _p(y), Nl, N, group, ntup,
@@ -174,28 +204,26 @@ def logsimpson(T, f):
     return lib.orc_logsimpson(_p(T), _p(f), len(T))
 
 
-def leff_groups(geom, fermion, T_eff, cx, cy, gsum, counts, policy=1):
-    """Oracle L_eff per group (Tfunc policy + log-Simpson), /(-2pi) or /(4pi)."""
-    rho2 = cx * cx + cy * cy
-    rho = np.sqrt(rho2)
+def leff_groups(geom, fermion, T, cx, cy, gsum, counts, policy=1):
+    """Oracle L_eff per group: Tfunc's branch policy at each node's effective
+    parameters, composite Simpson in ln T over the nominal nodes, /(-2pi)
+    scalar or /(4pi) spinor (Tint.c:69-72)."""
     ng = gsum.shape[1]
     out = np.zeros(ng)
+    f = np.zeros(len(T))
     for q in range(ng):
-        f = np.zeros(len(T_eff))
-        for k, T in enumerate(T_eff):
+        for k, Tn in enumerate(T):
+            ex, ey, rt = geom.tuple(cx, cy, Tn)
+            t = rt * rt
+            rho2 = ex * ex + ey * ey
             g3 = gsum[k, q]
             evm1 = (8.0 / 3.0 * g3[0] - 2.0 * g3[1] + 1.0 / 3.0 * g3[2]) / counts[q]
-            if policy:
-                v, br = igrand(geom, fermion, T, rho, rho2, 1.0 + evm1)
-                if br == 3:
-                    B = lib.orc_magfield(geom.ref, rho2)
-                    TB = 0.30282212 * T * B
-                    v = np.exp(-T) / T**3 * (evm1 - TB * TB / 3.0 if fermion else evm1 + TB * TB / 6.0)
-            else:
+            v, br = igrand(geom, fermion, t, np.sqrt(rho2), rho2, 1.0 + evm1)
+            if br == 3 or not policy:
                 B = lib.orc_magfield(geom.ref, rho2)
-                TB = 0.30282212 * T * B
-                v = np.exp(-T) / T**3 * (evm1 - TB * TB / 3.0 if fermion else evm1 + TB * TB / 6.0)
+                TB = 0.30282212 * t * B
+                v = np.exp(-t) / t**3 * (evm1 - TB * TB / 3.0 if fermion else evm1 + TB * TB / 6.0)
             f[k] = v
-        I = logsimpson(T_eff, f)
+        I = logsimpson(np.asarray(T, np.float64), f)
         out[q] = I / (4 * np.pi) if fermion else I / (-2 * np.pi)
     return out

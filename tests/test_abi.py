@@ -91,10 +91,22 @@ def test_gen_loops_and_text_reader_roundtrip(wfl, tmp_path):
     assert wfl.lib.wfl_read_text_loops(str(p).encode(), 4, 8, rx.ctypes.data, ry.ctypes.data) == -1
 
 
-def test_effective_T_and_group_ev(wfl):
-    for T in (1e-3, 0.37, 12.0):
-        r = np.float32(np.sqrt(T))
-        assert wfl.effective_T(T)[0] == float(r) * float(r)
+def test_effective_parameters_match_oracle(wfl, orc):
+    """The engine's parameter narrowing (FP32 in the Gaussian fast path's
+    scaled units, plain FP32 otherwise) restated independently in the
+    oracle's binding gives the same doubles."""
+    for kind, l2, F, lat in [(wfl.GAUSSIAN, 0.25, 10.0, 0.0), (wfl.GAUSSIAN, 1.7, 0.1, 1.3), (wfl.SMOOTH, 0.7, 2.0, 0.0)]:
+        g = wfl.Geometry(kind=kind, F=F, l2=l2, lattice=lat, images=1)
+        og = orc.Geom(kind, F=F, l2=l2, lattice=lat, images=1)
+        for cx, cy, T in [(0.0, 0.0, 1e-3), (0.37, -1.1, 0.77), (3.3, 0.2, 96.0)]:
+            e = wfl.effective_tuple(g, cx, cy, T)
+            assert (e[0], e[1], e[5]) == (og.c.l2, og.c.F, og.c.lattice)
+            assert tuple(e[2:5]) == og.tuple(cx, cy, T)
+            assert wfl.effective_T(g, T)[0] == e[4] ** 2
+            assert abs(e[4] ** 2 / T - 1) < 2e-7 and abs(e[0] / l2 - 1) < 2e-7
+
+
+def test_group_ev(wfl):
     g = np.array([3.0, -1.5, 0.6])
     assert wfl.group_ev_minus1(g, 3) == pytest.approx((8 / 3 * 3.0 + 2 * 1.5 + 0.2) / 3)
 
@@ -103,7 +115,7 @@ def test_leff_groups_matches_oracle(wfl, orc):
     """Host L_eff (Tfunc policy + log-Simpson) against the oracle's on
     synthetic group sums."""
     rng = np.random.default_rng(1)
-    T = wfl.effective_T(np.logspace(-3, 2, 24))
+    T = np.logspace(-3, 2, 24)
     gs = -np.abs(rng.normal(size=(24, 4, 3))) * 1e-2
     counts = np.full(4, 128, np.int32)
     for fermion in (0, 1):

@@ -61,7 +61,7 @@ def make_params(wfl, d_wl, d_s, d_f, hs, hf, F, l2, rho, fermion, ng=2):
 
 
 def oracle_float4(orc, x, y, F, l2, rho, T, fermion, mode=0, nsub=50):
-    g = orc.Geom(orc.GAUSSIAN, F=F, l2=l2)      # F and l2 narrowed to float as intEV.cu:124 does
+    g = orc.Geom(orc.GAUSSIAN, F=F, l2=l2, quantize="float")   # F, l2 narrowed to float (intEV.cu:124)
     rt = float(np.float32(np.sqrt(T)))
     ws = np.zeros((NL, 4), np.float32)
     wf = np.zeros((NL, 4), np.float32)

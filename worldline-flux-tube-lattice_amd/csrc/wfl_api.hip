@@ -68,7 +68,7 @@ static int require_device() {
 }
 
 // ------------------------------------------------------ data kernels
-// Loop records (see wfl_kernels.h): w, dw = w_succ - w, e = w_succ - w_pred.
+// Loop records (see wfl_kernels.h): w, 4 dw (dw = w_succ - w), e = w_succ - w_pred.
 // Differences of FP32 inputs are exact in double and rounded once to R.
 template <class R>
 __global__ void k_records(const float* __restrict__ x, const float* __restrict__ y, int Nl, int N,
@@ -83,8 +83,8 @@ __global__ void k_records(const float* __restrict__ x, const float* __restrict__
   R* r = rec + (size_t)idx * REC;
   r[0] = (R)wx;
   r[1] = (R)wy;
-  r[2] = (R)((double)xl[s] - wx);
-  r[3] = (R)((double)yl[s] - wy);
+  r[2] = (R)(4.0 * ((double)xl[s] - wx));   // 4 dw: exact scaling, carries Simpson's midpoint weight
+  r[3] = (R)(4.0 * ((double)yl[s] - wy));
   r[4] = (R)((double)xl[s] - (double)xl[p]);
   r[5] = (R)((double)yl[s] - (double)yl[p]);
   r[6] = (R)0;
@@ -235,23 +235,65 @@ static int loops_records(wfl_loops* L, int precision) {
 }
 
 // ------------------------------------------------------ geometry upload
-static void fill_geom(const wfl_geometry& g, int spline_off, int spline_n, DevGeom& d) {
+static const double WFL_LOG2E = 1.4426950408889634074;
+
+// Unit scale of the FP32 kernel for geometry g: sigma = sqrt(log2(e)/l2)
+// rounded to FP32 on the Gaussian fast path (node = exp2(-u'), 1/u'), 1 for
+// the generic profiles.  The effective problem parameters are defined from
+// the FP32 values so that the FP32 and FP64 kernels evaluate the same problem.
+static double unit_scale(const wfl_geometry& g, bool scaled) {
+  if (!scaled || g.kind != WFL_GAUSSIAN) return 1.0;
+  return (double)(float)sqrt(WFL_LOG2E / g.l2);
+}
+
+static void fill_geom(const wfl_geometry& g, int spline_off, int spline_n, bool scaled, DevGeom& d) {
   memset(&d, 0, sizeof d);
+  const double sg = unit_scale(g, scaled);
   d.kind = g.kind;
   d.images = g.images;
   d.spline_off = spline_off;
   d.spline_n = spline_n;
   d.Ff = (float)g.F;       d.F = d.Ff;
-  d.l2f = (float)g.l2;     d.l2 = d.l2f;
+  d.l2 = (sg != 1.0) ? WFL_LOG2E / (sg * sg) : (double)(float)g.l2;
+  d.l2f = (float)d.l2;
   d.inv_l2 = 1.0 / d.l2;   d.inv_l2f = (float)d.inv_l2;
   d.af = (float)g.a;       d.a = d.af;
   d.lminf = (float)g.lam_min; d.lmin = d.lminf;
   d.q1f = (float)g.fix_q1; d.q1 = d.q1f;
   d.q2f = (float)g.fix_q2; d.q2 = d.q2f;
-  d.latticef = (float)g.lattice; d.lattice = g.lattice > 0.0 ? (double)d.latticef : 0.0;
-  double k = -1.4426950408889634074 / d.l2;
-  d.k_hi = (float)k;
-  d.k_lo = (float)(k - (double)d.k_hi);
+  if (g.lattice > 0.0) {
+    d.latticef = (float)(sg * g.lattice);
+    d.lattice = (double)d.latticef / sg;
+  }
+  d.c8f = (float)(1.0e-8 * sg * sg);
+  d.c10f = (float)(1.0e-10 * sg * sg);
+}
+
+static void fill_tuple(double sg, const DevGeom& dg, int gi, double cx, double cy, double rtT, int nloops,
+                       DevTuple& d) {
+  memset(&d, 0, sizeof d);
+  d.geom = gi;
+  d.nloops = nloops;
+  d.cx = (float)(sg * cx);
+  d.cy = (float)(sg * cy);
+  d.s = (float)(sg * rtT);
+  d.cxd = (double)d.cx / sg;
+  d.cyd = (double)d.cy / sg;
+  d.sd = (double)d.s / sg;
+  if (dg.lattice > 0.0) {
+    d.m0 = (int)floor(d.cxd / dg.lattice + 0.5);
+    d.n0 = (int)floor(d.cyd / dg.lattice + 0.5);
+  }
+}
+
+extern "C" int wfl_effective_tuple(const wfl_geometry* g, double cx, double cy, double T, double out[6]) {
+  if (!g || !out || !(g->l2 > 0.0) || !(T > 0.0)) { wfl_set_error("wfl_effective_tuple: bad arguments"); return WFL_ERR_ARG; }
+  DevGeom dg;
+  DevTuple dt;
+  fill_geom(*g, 0, 0, true, dg);
+  fill_tuple(unit_scale(*g, true), dg, 0, cx, cy, sqrt(T), 0, dt);
+  out[0] = dg.l2; out[1] = dg.F; out[2] = dt.cxd; out[3] = dt.cyd; out[4] = dt.sd; out[5] = dg.lattice;
+  return WFL_OK;
 }
 
 static int check_geom(const wfl_geometry& g, int nspline_tables, const wfl_spline_table* tabs) {
@@ -365,27 +407,17 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
     const wfl_geometry& G = b->geoms[g];
     int off = 0, n = 0;
     if (G.kind == WFL_SPLINE) { off = sp_off[G.spline_table]; n = b->splines[G.spline_table].n; }
-    fill_geom(G, off, n, geoms[g]);
+    fill_geom(G, off, n, true, geoms[g]);
     if (G.kind != WFL_GAUSSIAN) p->prof_gauss = 0;
   }
-  // tuples (FP32-quantized parameters, image centre in double)
+  // tuples: FP32 kernel-unit values and the effective doubles
   p->tups.resize(b->ntup);
   for (int t = 0; t < b->ntup; ++t) {
-    DevTuple& d = p->tups[t];
-    const DevGeom& G = geoms[b->tup_geom[t]];
-    d.geom = b->tup_geom[t];
-    d.cx = (float)b->tup_cx[t];
-    d.cy = (float)b->tup_cy[t];
-    d.rtT = (float)sqrt(b->tup_T[t]);
-    d.pad = 0.0f;
-    d.m0 = d.n0 = 0;
-    if (G.lattice > 0.0) {
-      d.m0 = (int)floor((double)d.cx / G.lattice + 0.5);
-      d.n0 = (int)floor((double)d.cy / G.lattice + 0.5);
-    }
+    const int gi = b->tup_geom[t];
     int nl = Nl;
-    if (b->geom_nloops) nl = std::max(0, std::min(Nl, b->geom_nloops[d.geom]));
-    d.nloops = nl;
+    if (b->geom_nloops) nl = std::max(0, std::min(Nl, b->geom_nloops[gi]));
+    fill_tuple(unit_scale(b->geoms[gi], true), geoms[gi], gi, b->tup_cx[t], b->tup_cy[t], sqrt(b->tup_T[t]), nl,
+               p->tups[t]);
   }
   // lane groups: tuples sorted by geometry, each geometry padded to 64 lanes
   std::vector<int> order(b->ntup);
@@ -599,12 +631,16 @@ extern "C" int wfl_internal_ev_launch(const void* dev_aos, int Nl, int N, const 
   }
   int rc = loops_records(g_ev.L, precision);
   if (rc) return rc;
+  // FP64: the reference's float parameters as they are; FP32 Gaussian:
+  // the same parameters in the scaled units of the fast path.
+  wfl_geometry gg = *g;
+  gg.lattice = 0.0;
+  gg.images = 0;
+  const bool scaled = precision == WFL_MIXED;
   DevGeom dg;
-  fill_geom(*g, 0, g->kind == WFL_SPLINE ? nspline : 0, dg);
-  dg.lattice = 0.0; dg.latticef = 0.0f; dg.images = 0;
+  fill_geom(gg, 0, g->kind == WFL_SPLINE ? nspline : 0, scaled, dg);
   DevTuple dt;
-  memset(&dt, 0, sizeof dt);
-  dt.geom = 0; dt.cx = cx; dt.cy = cy; dt.rtT = rtT; dt.nloops = Nl;
+  fill_tuple(unit_scale(gg, scaled), dg, 0, (double)cx, (double)cy, (double)rtT, Nl, dt);
   HIP_TRY(hipMemcpy(g_ev.d_geom, &dg, sizeof dg, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(g_ev.d_tup, &dt, sizeof dt, hipMemcpyHostToDevice));
   WflLaunch a;

@@ -366,28 +366,46 @@ double wfl_logsimpson(const double *T, const double *f, int n) {
 
 /* L_eff(rho) per loop group from batched group sums on a log T grid.
  * gsum: [nT][ngroups][3] sums of W - 1 (the tuples of one (geometry, x_cm)
- * in T order); counts: loops per group.  policy 1 applies the reference's
- * analytic small-/large-T branches, policy 0 uses the worldline estimate at
- * every node.  T must be the effective proper times the kernels used. */
+ * in T order); counts: loops per group.  g, T, cx, cy are the values the
+ * batch was created with; each node's integrand uses the effective
+ * parameters the kernels evaluated (wfl_effective_tuple), the quadrature the
+ * nominal log-spaced nodes.  policy 1 applies the reference's analytic
+ * small-/large-T branches, policy 0 the worldline estimate at every node. */
 int wfl_leff_groups(const wfl_geometry *g, int fermion, int nT, const double *T, double cx, double cy,
                     const double *gsum, const int *counts, int ngroups, int policy, double *leff) {
   if (!g || !T || !gsum || !counts || !leff || nT < 2 || ngroups < 1) { wfl_set_error("wfl_leff_groups: bad arguments"); return WFL_ERR_ARG; }
-  wfl_geometry gq = *g;
-  gq.F = wfl_q32(g->F);
-  gq.l2 = wfl_q32(g->l2);
-  double rho2 = cx * cx + cy * cy, rho = sqrt(rho2);
-  double *f = (double *)malloc(sizeof(double) * (size_t)nT);
+  double *f = (double *)malloc(sizeof(double) * 5 * (size_t)nT);
   if (!f) { wfl_set_error("wfl_leff_groups: malloc"); return WFL_ERR_NOMEM; }
+  double *Te = f + nT, *B = f + 2 * nT, *sT = f + 3 * nT, *rr = f + 4 * nT;
+  wfl_geometry gq = *g;
+  for (int k = 0; k < nT; ++k) {
+    double e[6];
+    int rc = wfl_effective_tuple(g, cx, cy, T[k], e);
+    if (rc) { free(f); return rc; }
+    gq.l2 = e[0];
+    gq.F = e[1];
+    Te[k] = e[4] * e[4];
+    rr[k] = e[2] * e[2] + e[3] * e[3];
+    B[k] = magfield_geom(&gq, rr[k]);
+    sT[k] = smallT_geom(&gq, sqrt(rr[k]));
+  }
   for (int q = 0; q < ngroups; ++q) {
     for (int k = 0; k < nT; ++k) {
-      int branch;
       double evm1 = wfl_group_ev_minus1(gsum + ((size_t)k * ngroups + q) * 3, counts[q]);
-      double v = igrand_geom(&gq, fermion, T[k], rho, rho2, evm1, &branch);
-      if (!policy && branch != 3) {
-        double B = magfield_geom(&gq, rho2), TB = WFL_E * T[k] * B;
-        v = (fermion == 1) ? exp(-WFL_M2 * T[k]) / (T[k] * T[k] * T[k]) * (evm1 - TB * TB / 3.0)
-                           : exp(-WFL_M2 * T[k]) / (T[k] * T[k] * T[k]) * (evm1 + TB * TB / 6.0);
+      double t = Te[k], TB = WFL_E * t * B[k], v;
+      int wl = 1;
+      if (policy) {
+        if (t < 0.1 && t < 0.01 * sT[k]) { v = fermion == 1 ? smTferm(B[k], t) : smTscal(B[k], t); wl = 0; }
+        else if (t < 0.01 * sT[k]) { v = TB > 1.0e-6 ? Exact(t, B[k], fermion) : 0.0; wl = 0; }
+        else if (t > 49.0) {
+          v = (fermion == 1) ? -exp(-WFL_M2 * t) / (3.0 * t) * WFL_E * WFL_E * B[k] * B[k]
+                             : exp(-WFL_M2 * t) / (6.0 * t) * WFL_E * WFL_E * B[k] * B[k];
+          wl = 0;
+        }
       }
+      if (wl)
+        v = (fermion == 1) ? exp(-WFL_M2 * t) / (t * t * t) * (evm1 - TB * TB / 3.0)
+                           : exp(-WFL_M2 * t) / (t * t * t) * (evm1 + TB * TB / 6.0);
       f[k] = v;
     }
     double I = wfl_logsimpson(T, f, nT);
@@ -397,9 +415,10 @@ int wfl_leff_groups(const wfl_geometry *g, int fermion, int nT, const double *T,
   return WFL_OK;
 }
 
-double wfl_effective_T(double T) {
-  double r = (double)(float)sqrt(T);
-  return r * r;
+double wfl_effective_T(const wfl_geometry *g, double T) {
+  double e[6];
+  if (wfl_effective_tuple(g, 0.0, 0.0, T, e)) return NAN;
+  return e[4] * e[4];
 }
 
 /* ------------------------------------------------------------------ */

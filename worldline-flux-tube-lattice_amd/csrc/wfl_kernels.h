@@ -2,7 +2,7 @@
 // arithmetic type; instantiated in wfl_kernels_f32.hip / wfl_kernels_f64.hip).
 //
 // Data: each loop point i carries a record of six values
-//   (w_x, w_y, dw_x, dw_y, e_x, e_y),  dw = w_succ(i) - w_i,  e = w_succ(i) - w_pred(i)
+//   (w_x, w_y, 4 dw_x, 4 dw_y, e_x, e_y),  dw = w_succ(i) - w_i,  e = w_succ(i) - w_pred(i)
 // where succ/pred follow the sub-loop map of getzp1 (intEV.cu:462-506).
 // Records are stored point-major inside each loop ([Nl][N][REC], REC = 8: six values and two pad slots, 32-byte aligned) so that one
 // wavefront walking one loop reads them with scalar loads.
@@ -45,12 +45,13 @@ __device__ __forceinline__ int pred_idx(int i, int N) {
 // accumulator slot of point i: 0 = x (odd), 1 = y (2 mod 4), 2 = z (0 mod 4)
 __device__ __forceinline__ int cls_of(int i) { return (i & 1) ? 0 : ((i & 3) == 0 ? 2 : 1); }
 
-// Per-lane constants of one tuple in the kernel's type.
+// Per-lane constants of one tuple in the kernel's type and units.
 template <class R> struct LaneC {
-  R s, hs;                 // sqrt(T), sqrt(T)/2
-  R inv_l2;
-  float k_hi, k_lo;
-  R spin_scale;            // 1/l2 on the Gaussian fast path (f' = e/l2), else 1
+  R s;          // sqrt(T)
+  R m8;         // s/8: midpoint offset per unit of the stored 4 dw
+  R s4;         // s/4: segment vector per unit of the stored 4 dw
+  R inv_l2;     // 1/l2 (FP64 Gaussian node)
+  R c8, c10;    // the reference clamps 1e-8, 1e-10 in kernel units
 };
 
 // --------------------------------------------------------- node evaluation
@@ -60,7 +61,7 @@ __device__ __forceinline__ R node_g(R u, const LaneC<R>& lc, int kind, const GP<
                                     bool spinor, R& fp) {
   if (PROF == P_GAUSSIAN) {
     R e;
-    R gval = gauss_g(u, lc.inv_l2, lc.k_hi, lc.k_lo, e);
+    R gval = gauss_g(u, (double)lc.inv_l2, e);
     fp = e;
     return gval;
   } else {
@@ -68,73 +69,75 @@ __device__ __forceinline__ R node_g(R u, const LaneC<R>& lc, int kind, const GP<
   }
 }
 
-// One point (= one segment start) for one image centre.  Adds to P (phase)
-// and S (spin) of the point's class; all terms are before the common
-// factors F s h/3 (phase) and h/3 * spin_scale (spin).
+// One point (= one segment start) for one image centre.  The record holds
+// 4 dw (an exact power-of-two scaling), so X = 4 z x dw carries the Simpson
+// midpoint weight.  Adds to P (phase) and S (half the spin sum) of the
+// point's class, before the common factors F s h/3 (phase) and
+// 2 h/3 spin_scale (spin), h = 1/n.
 template <class R, int PROF, int SPINOR, int MODE, int NSUB>
-__device__ __forceinline__ void point_eval(R wx, R wy, R dx, R dy, R ex, R ey,
+__device__ __forceinline__ void point_eval(R wx, R wy, R dx4, R dy4, R ex, R ey,
                                            R ccx, R ccy, const LaneC<R>& lc, int kind, const GP<R>& gp,
                                            const float* __restrict__ kn, const float4* __restrict__ cf, int ns,
                                            int nsub_rt, R& P, R& S) {
   const int n = NSUB ? NSUB : nsub_rt;
+  const R tiny = sizeof(R) == 4 ? (R)1e-30 : (R)0;   // keeps 1/u finite on the axis (FP32 node)
   R zx = fma(lc.s, wx, ccx);
   R zy = fma(lc.s, wy, ccy);
-  R X = fma(zx, dy, -(zy * dx));          // z x dw   (z x z' = s (z x dw))
+  R X = fma(zx, dy4, -(zy * dx4));        // 4 z x dw
   if (MODE == M_PHYSICAL) {
-    R E = fma(zx, ey, -(zy * ex));        // z x e    (z x (z'-z_prev) / s)
-    R A = fma(zx, zx, zy * zy);
-    R fpA, fpk;
-    R gA = node_g<R, PROF>(A, lc, kind, gp, MODE, kn, cf, ns, SPINOR, fpA);
-    R gin = (R)0, fin = (R)0;
+    R E = fma(zx, ey, -(zy * ex));        // z x e: both endpoint weights of this point
+    R A = fma(zx, zx, fma(zy, zy, tiny));
+    R fA, fk;
+    R gA = node_g<R, PROF>(A, lc, kind, gp, MODE, kn, cf, ns, SPINOR, fA);
+    R gin, fin;
     if (n == 2) {
-      R mx = fma(lc.hs, dx, zx), my = fma(lc.hs, dy, zy);
-      R Am = fma(mx, mx, my * my);
-      gin = (R)4 * node_g<R, PROF>(Am, lc, kind, gp, MODE, kn, cf, ns, SPINOR, fpk);
-      fin = (R)4 * fpk;
+      R mx = fma(lc.m8, dx4, zx), my = fma(lc.m8, dy4, zy);
+      R Am = fma(mx, mx, fma(my, my, tiny));
+      gin = node_g<R, PROF>(Am, lc, kind, gp, MODE, kn, cf, ns, SPINOR, fk);
+      fin = fk + fk;
     } else {
-      const R h = (R)1 / (R)n;
+      const R h4 = lc.s4 / (R)n;          // node k sits at z + (k h s/4)(4 dw)
+      gin = (R)0; fin = (R)0;
       for (int k = 1; k < n; ++k) {
-        R t = (R)k * h * lc.s;
-        R px = fma(t, dx, zx), py = fma(t, dy, zy);
-        R u = fma(px, px, py * py);
-        R w = (k & 1) ? (R)4 : (R)2;
-        R gk = node_g<R, PROF>(u, lc, kind, gp, MODE, kn, cf, ns, SPINOR, fpk);
-        gin = fma(w, gk, gin);
-        if (SPINOR) fin = fma(w, fpk, fin);
+        R t = (R)k * h4;
+        R px = fma(t, dx4, zx), py = fma(t, dy4, zy);
+        R u = fma(px, px, fma(py, py, tiny));
+        R gk = node_g<R, PROF>(u, lc, kind, gp, MODE, kn, cf, ns, SPINOR, fk);
+        gin = (k & 1) ? gin + gk : fma((R)0.5, gk, gin);      // Simpson weights 4, 2 over 4
+        if (SPINOR) fin = (k & 1) ? fma((R)2, fk, fin) : fin + fk;   // and over 2
       }
     }
     P = fma(E, gA, fma(X, gin, P));
-    if (SPINOR) S += fma((R)2, fpA, fin);
+    if (SPINOR) S += fA + fin;
   } else {
     // Reference Idt: rho^2(t) = A + 2Bt + Ct^2 with the clamps of intEV.cu:418-422, 439.
-    R dsx = lc.s * dx, dsy = lc.s * dy;
+    R dsx = lc.s4 * dx4, dsy = lc.s4 * dy4;   // z' - z
     R A = fma(zx, zx, zy * zy);
     R B = fma(zx, dsx, zy * dsy);
     R C = fma(dsx, dsx, dsy * dsy);
-    const R c8 = (R)1.0e-8, c10 = (R)1.0e-10;
-    A = A < c8 ? c8 : A;
-    B = B < c8 ? c8 : B;
-    C = C < c8 ? c8 : C;
+    A = A < lc.c8 ? lc.c8 : A;
+    B = B < lc.c8 ? lc.c8 : B;
+    C = C < lc.c8 ? lc.c8 : C;
     R A1 = A + (R)2 * B + C;
-    A1 = A1 < c8 ? c8 : A1;
+    A1 = A1 < lc.c8 ? lc.c8 : A1;
     R f0, f1, fk;
     R g0 = node_g<R, PROF>(A, lc, kind, gp, MODE, kn, cf, ns, SPINOR, f0);
     R g1 = node_g<R, PROF>(A1, lc, kind, gp, MODE, kn, cf, ns, SPINOR, f1);
-    R gs = g0 + g1, fs = SPINOR ? f0 + f1 : (R)0;
+    R gs = (R)0.25 * (g0 + g1), fs = SPINOR ? (R)0.5 * (f0 + f1) : (R)0;
     if (n == 2) {
       R u = A + B + (R)0.25 * C;
-      u = u < c10 ? c10 : u;
-      gs = fma((R)4, node_g<R, PROF>(u, lc, kind, gp, MODE, kn, cf, ns, SPINOR, fk), gs);
-      if (SPINOR) fs = fma((R)4, fk, fs);
+      u = u < lc.c10 ? lc.c10 : u;
+      gs += node_g<R, PROF>(u, lc, kind, gp, MODE, kn, cf, ns, SPINOR, fk);
+      if (SPINOR) fs = fma((R)2, fk, fs);
     } else {
       const R h = (R)1 / (R)n;
       for (int k = 1; k < n; ++k) {
         R t = (R)k * h;
         R u = A + (R)2 * B * t + C * t * t;
-        u = u < c10 ? c10 : u;
-        R w = (k & 1) ? (R)4 : (R)2;
-        gs = fma(w, node_g<R, PROF>(u, lc, kind, gp, MODE, kn, cf, ns, SPINOR, fk), gs);
-        if (SPINOR) fs = fma(w, fk, fs);
+        u = u < lc.c10 ? lc.c10 : u;
+        R gk = node_g<R, PROF>(u, lc, kind, gp, MODE, kn, cf, ns, SPINOR, fk);
+        gs = (k & 1) ? gs + gk : fma((R)0.5, gk, gs);
+        if (SPINOR) fs = (k & 1) ? fma((R)2, fk, fs) : fs + fk;
       }
     }
     P = fma(X, gs, P);
@@ -174,25 +177,28 @@ __device__ __forceinline__ void wminus1(int mode, bool spinor, const double sc[3
 }
 
 template <class R>
-__device__ __forceinline__ void lane_setup(const DevTuple& tp, const DevGeom& gm, int PROF, LaneC<R>& lc) {
-  lc.s = (R)tp.rtT;
-  lc.hs = (R)0.5 * lc.s;
-  lc.inv_l2 = sizeof(R) == 4 ? (R)gm.inv_l2f : (R)gm.inv_l2;
-  lc.k_hi = gm.k_hi;
-  lc.k_lo = gm.k_lo;
-  lc.spin_scale = (PROF == P_GAUSSIAN) ? lc.inv_l2 : (R)1;
+__device__ __forceinline__ void lane_setup(const DevTuple& tp, const DevGeom& gm, LaneC<R>& lc) {
+  const bool f32 = sizeof(R) == 4;
+  lc.s = f32 ? (R)tp.s : (R)tp.sd;
+  lc.m8 = (R)0.125 * lc.s;
+  lc.s4 = (R)0.25 * lc.s;
+  lc.inv_l2 = (R)gm.inv_l2;
+  lc.c8 = f32 ? (R)gm.c8f : (R)1.0e-8;
+  lc.c10 = f32 ? (R)gm.c10f : (R)1.0e-10;
 }
 
-// Per-loop outputs from the class sums (FP64).
+// Per-loop outputs from the class sums (FP64): Wsscal = F s (h/3) P and
+// Wsferm = 2 F T (2 h/3 spin_scale S) / N_class (intEV.cu:553-560), where s
+// is sqrt(T) in the kernel's units and T the effective proper time.
 __device__ __forceinline__ void finalize_loop(const double P[3], const double S[3], double F, double s,
-                                              double spin_scale, int n, int N, bool spinor,
+                                              double T, double spin_scale, int n, int N, bool spinor,
                                               double sc[3], double fe[3]) {
   const double h3 = 1.0 / (3.0 * (double)n);
   sc[0] = F * s * h3 * P[0];
   sc[1] = F * s * h3 * P[1];
   sc[2] = F * s * h3 * P[2];
   if (spinor) {
-    const double k = 2.0 * F * s * s * h3 * spin_scale;
+    const double k = 4.0 * F * T * h3 * spin_scale;
     fe[0] = k * S[0] / (0.5 * N);
     fe[1] = k * S[1] / (0.25 * N);
     fe[2] = k * S[2] / (0.25 * N);
@@ -226,7 +232,9 @@ __global__ __launch_bounds__(256) void k_tuple(
   const DevGeom gm = geoms[tp.geom];
   const GP<R> gp(gm);
   LaneC<R> lc;
-  lane_setup<R>(tp, gm, PROF, lc);
+  lane_setup<R>(tp, gm, lc);
+  const double s_phase = sizeof(R) == 4 ? (double)tp.s : tp.sd;
+  const double spin_scale = (PROF == P_GAUSSIAN) ? gm.inv_l2 : 1.0;
   const int kind = gm.kind;
   const float* kn = knots + gm.spline_off;
   const float4* cf = coefs + gm.spline_off;
@@ -235,7 +243,7 @@ __global__ __launch_bounds__(256) void k_tuple(
   // image window (uniform over the lane group: one geometry per group)
   const int M = __builtin_amdgcn_readfirstlane(gm.lattice > 0.0 ? gm.images : 0);
   const R lat = sizeof(R) == 4 ? (R)gm.latticef : (R)gm.lattice;
-  const R cx = (R)tp.cx, cy = (R)tp.cy;
+  const R cx = sizeof(R) == 4 ? (R)tp.cx : (R)tp.cxd, cy = sizeof(R) == 4 ? (R)tp.cy : (R)tp.cyd;
 
   double G[3] = {0.0, 0.0, 0.0};
   for (int l = loop0; l < loop0 + nloop; ++l) {
@@ -267,7 +275,7 @@ __global__ __launch_bounds__(256) void k_tuple(
       }
     }
     double sc[3], fe[3], wm[3];
-    finalize_loop(P, S, gm.F, (double)tp.rtT, (double)lc.spin_scale, n, N, SPINOR, sc, fe);
+    finalize_loop(P, S, gm.F, s_phase, tp.sd * tp.sd, spin_scale, n, N, SPINOR, sc, fe);
     wminus1(MODE, SPINOR, sc, fe, wm);
     bool ok = finite_r(wm[0]) && finite_r(wm[1]) && finite_r(wm[2]);
     if (valid && !ok) atomicAdd(nonfinite, 1ull);
@@ -306,7 +314,9 @@ __global__ __launch_bounds__(256) void k_loop(
   const DevGeom gm = geoms[tp.geom];
   const GP<R> gp(gm);
   LaneC<R> lc;
-  lane_setup<R>(tp, gm, PROF, lc);
+  lane_setup<R>(tp, gm, lc);
+  const double s_phase = sizeof(R) == 4 ? (double)tp.s : tp.sd;
+  const double spin_scale = (PROF == P_GAUSSIAN) ? gm.inv_l2 : 1.0;
   const int kind = gm.kind;
   const float* kn = knots + gm.spline_off;
   const float4* cf = coefs + gm.spline_off;
@@ -314,7 +324,7 @@ __global__ __launch_bounds__(256) void k_loop(
   const int n = NSUB ? NSUB : nsub_rt;
   const int M = gm.lattice > 0.0 ? gm.images : 0;
   const R lat = sizeof(R) == 4 ? (R)gm.latticef : (R)gm.lattice;
-  const R cx = (R)tp.cx, cy = (R)tp.cy;
+  const R cx = sizeof(R) == 4 ? (R)tp.cx : (R)tp.cxd, cy = sizeof(R) == 4 ? (R)tp.cy : (R)tp.cyd;
   const R* __restrict__ rl = rec + (size_t)l * N * REC;
 
   double P = 0.0, S = 0.0;      // this lane's class: lane & 3 (points i = lane + 64k)
@@ -351,7 +361,7 @@ __global__ __launch_bounds__(256) void k_loop(
     double Pc[3] = {P1 + P3, P2, P};
     double Sc[3] = {S1 + S3, S2, S};
     double sc[3], fe[3], wm[3];
-    finalize_loop(Pc, Sc, gm.F, (double)tp.rtT, (double)lc.spin_scale, n, N, SPINOR, sc, fe);
+    finalize_loop(Pc, Sc, gm.F, s_phase, tp.sd * tp.sd, spin_scale, n, N, SPINOR, sc, fe);
     wminus1(MODE, SPINOR, sc, fe, wm);
     if (!(finite_r(wm[0]) && finite_r(wm[1]) && finite_r(wm[2]))) atomicAdd(nonfinite, 1ull);
     const size_t o = (size_t)t * Nl + l;

@@ -19,18 +19,28 @@ enum { P_ANY = 99 };
 
 // One geometry as the kernels see it.  Every double field holds a value
 // that is exactly representable in FP32.
+//
+// Coordinates.  The FP64 kernels and the FP32 kernels of the generic
+// profiles work in the problem's own units.  The FP32 Gaussian kernel works
+// in scaled units z' = sigma z with sigma = sqrt(log2(e)/l2) rounded to FP32,
+// so that a node needs only exp2(-u') and 1/u'; the host then defines the
+// problem's effective parameters from the FP32 values (l2 = log2(e)/sigma^2,
+// x_cm = c'/sigma, sqrt(T) = s'/sigma, lattice = L'/sigma), which is what the
+// FP64 kernels and the oracle evaluate.
 struct DevGeom {
   int kind, images, spline_off, spline_n;
-  double F, l2, inv_l2, a, lmin, q1, q2, lattice;
-  float Ff, l2f, inv_l2f, af, lminf, q1f, q2f, latticef;
-  float k_hi, k_lo;        // -log2(e)/l2 split into hi + lo
+  double F, l2, inv_l2, a, lmin, q1, q2, lattice;   // effective values (FP64 kernels)
+  float Ff, l2f, inv_l2f, af, lminf, q1f, q2f;      // FP32 generic-profile values
+  float latticef;                                   // lattice spacing in FP32 kernel units
+  float c8f, c10f;                                  // reference clamps 1e-8, 1e-10 in FP32 kernel units
 };
 
-// One evaluation tuple (x_cm, sqrt(T), geometry); the image-window centre
-// (m0, n0) is computed on the host in double.
+// One evaluation tuple; the image-window centre (m0, n0) is computed on the
+// host from the effective doubles.
 struct DevTuple {
   int geom, m0, n0, nloops;
-  float cx, cy, rtT, pad;
+  float cx, cy, s, padf;        // FP32 kernel units
+  double cxd, cyd, sd, padd;    // effective values (FP64 kernels)
 };
 
 // ---------------------------------------------------------------- math
@@ -69,20 +79,19 @@ template <> struct GP<double> {
 };
 
 // -------------------------------------------- Gaussian tube (hot profile)
-// FP32: returns g(u) = f(u)/u = (1 - e)/u and e = exp(-u/l2).  The exponent
-// is formed with -log2(e)/l2 split in hi + lo so that the rounding of that
-// constant does not bias every node the same way.  Near the tube axis
-// (u << l2) 1 - e carries the absolute rounding of e, i.e. a relative error
-// ~6e-8 l2/u; the phase weight of such a node is z x dw, which shrinks with
-// |z|, so the phase error per node stays below ~F |dw| 6e-8 / |z| (random in
-// sign).  u is floored at 1e-30: a node exactly on the axis has z x dw = 0.
-__device__ __forceinline__ float gauss_g(float u, float inv_l2, float k_hi, float k_lo, float& e) {
-  (void)inv_l2;
-  e = __builtin_amdgcn_exp2f(fmaf(u, k_hi, u * k_lo));
-  return (1.0f - e) * __builtin_amdgcn_rcpf(fmaxf(u, 1e-30f));
+// FP32, scaled units (u' = log2(e) rho^2 / l2): returns g' = (1 - e)/u' and
+// e = 2^-u' = exp(-rho^2/l2).  Near the tube axis 1 - e carries the absolute
+// rounding of e, a relative error ~6e-8/u'; the phase weight of such a node
+// is z x dw, which shrinks with |z|, so its phase error stays below
+// ~F |dw| 6e-8 / |z| and is random in sign.  The caller adds 1e-30 to u'
+// so that a node exactly on the axis (where z x dw = 0) gives 0, not NaN.
+__device__ __forceinline__ float gauss_g(float u, double, float& e) {
+  e = __builtin_amdgcn_exp2f(-u);
+  return (1.0f - e) * __builtin_amdgcn_rcpf(u);
 }
-// FP64: g(u) = -expm1(-u/l2)/u (limit 1/l2 at u = 0), e = 1 + expm1(-u/l2).
-__device__ __forceinline__ double gauss_g(double u, double inv_l2, float, float, double& e) {
+// FP64, problem units: g(u) = -expm1(-u/l2)/u (limit 1/l2 at u = 0),
+// e = 1 + expm1(-u/l2).
+__device__ __forceinline__ double gauss_g(double u, double inv_l2, double& e) {
   double x = u * inv_l2;
   double em1 = ::expm1(-x);
   e = 1.0 + em1;

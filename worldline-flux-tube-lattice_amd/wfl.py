@@ -82,6 +82,7 @@ EXPORTS = [
     "wfl_plan_gsum_device", "wfl_plan_nonfinite", "wfl_plan_mapping", "wfl_plan_destroy",
     "wfl_eval_batch", "wfl_group_ev_minus1", "wfl_spline_build", "wfl_set_profile",
     "wfl_set_numerics", "wfl_ev_reset", "wfl_ev_launches", "wfl_logsimpson", "wfl_effective_T",
+    "wfl_effective_tuple",
     "wfl_leff_groups",
     "EV", "EVMean", "Tfunc", "integrateT", "getwl", "Exact", "MagField", "smTscal", "smTferm",
 ]
@@ -121,7 +122,8 @@ def _load():
         "wfl_ev_reset": (None, []),
         "wfl_ev_launches": (C.c_longlong, []),
         "wfl_logsimpson": (D, [P, P, I]),
-        "wfl_effective_T": (D, [D]),
+        "wfl_effective_T": (D, [P, D]),
+        "wfl_effective_tuple": (I, [P, D, D, D, P]),
         "wfl_leff_groups": (I, [P, I, I, P, D, D, P, P, I, I, P]),
         "EV": (D, [D, P, P]),
         "EVMean": (None, [P, P, P, I, P, D, I]),
@@ -175,8 +177,18 @@ def spline_build(fli, rho2, quirk=False):
     return c4
 
 
-def effective_T(T):
-    return np.array([lib.wfl_effective_T(float(t)) for t in np.atleast_1d(T)])
+def effective_T(geom, T):
+    """Effective proper times the kernels evaluate for requested T values."""
+    g = geom.c()
+    return np.array([lib.wfl_effective_T(C.byref(g), float(t)) for t in np.atleast_1d(T)])
+
+
+def effective_tuple(geom, cx, cy, T):
+    """(l2, F, cx, cy, sqrt(T), lattice) a batch evaluates for one tuple."""
+    g = geom.c()
+    out = np.zeros(6)
+    check(lib.wfl_effective_tuple(C.byref(g), float(cx), float(cy), float(T), _ptr(out)))
+    return out
 
 
 class Loops:
@@ -341,9 +353,10 @@ def logsimpson(T, f):
     return lib.wfl_logsimpson(_ptr(T), _ptr(f), len(T))
 
 
-def leff_groups(geom: Geometry, fermion, T_eff, cx, cy, gsum, counts, policy=1):
-    """L_eff per group; gsum [nT, ngroups, 3] for one (geometry, x_cm)."""
-    T_eff = np.ascontiguousarray(T_eff, np.float64)
+def leff_groups(geom: Geometry, fermion, T, cx, cy, gsum, counts, policy=1):
+    """L_eff per group; gsum [nT, ngroups, 3] for one (geometry, x_cm); T,
+    cx, cy as the batch was created with (nominal, log-spaced T)."""
+    T_eff = np.ascontiguousarray(T, np.float64)
     gsum = np.ascontiguousarray(gsum, np.float64)
     counts = np.ascontiguousarray(counts, np.int32)
     ng = gsum.shape[1]

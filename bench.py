@@ -1,17 +1,26 @@
 """Benchmark of the worldline Wilson-loop hot path on one MI355X.
 
-One step = one batched evaluation of the spinor-QED workload (the engine's
-multi-geometry configuration): 5120 unit loops x 4096 points, 9 Gaussian
+Default (the bench line): one step = one batched evaluation of the
+multi-geometry spinor workload: 5120 unit loops x 4096 points, 9 Gaussian
 tubes (lambda in {0.5, 1, 2} x F in {0.1, 1, 10}), 64 x_cm values
-rho in [0, 4 lambda], 96 log-spaced T in [1e-3, 1e2], Simpson n = 2,
-physical segment mode, reduced on the device to FP64 sums of W - 1 per
-group of 128 loops.
+rho in [0, 4 lambda], 96 log-spaced T in [1e-3, 1e2], Simpson n = 2, physical
+segment mode, reduced on the device to FP64 sums of W - 1 per group of 128
+loops.  One point-eval = one (loop, point, x_cm, T, geometry) tuple.
 
-Prints one JSON line: throughput of the mixed-precision engine (point-evals/s,
-one point-eval = one (loop, point, x_cm, T, geometry) tuple), the FP64 engine
-on the same workload, the L_eff / <W> error of mixed against FP64, the
-roofline of the dominant kernel and a CPU baseline (the FP64 C oracle on a
-bounded sample).
+Prints one JSON line: mixed-precision throughput, the FP64 engine on the same
+workload, the L_eff / <W> error of mixed against FP64, the roofline of the
+dominant kernel and a CPU baseline (the FP64 C oracle on a bounded sample).
+
+--config selects the other survey workloads (one JSON line each):
+  c1  scalar, 512-knot spline of the Gaussian profile, 1024 x 1024 loops,
+      32 rho x 64 T; GPU FP64 and mixed, and the CPU oracle at full size
+  c3  dense lattice of Gaussian tubes, 5 x 5 images, x_cm on a 32 x 32 grid
+      over the unit cell, l in {0.1, 0.3, 1, 3, 10}; interaction energy per
+      tube against the isolated tube (--scale shrinks the loop count)
+  c4  48 uneven geometries (seed 42) from a shared 8192 x 2048 pool in one
+      persistent launch, autotuned and replayed as a hipGraph
+  c5  65536 x 8192 loops, 256 rho x 128 T, 3 x 3 images at spacing 2;
+      ensemble standard error against the precision error (--scale shrinks)
 """
 import argparse
 import json
@@ -24,90 +33,401 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Algorithmic FLOPs per point-eval (Gaussian tube, n Simpson nodes per segment,
-# K images): scalar 5 + K(21 + 9n), spinor 5 + K(27 + 11n).
-def flops_per_point_eval(spinor, n, K=1):
-    return 5 + K * ((27 + 11 * n) if spinor else (21 + 9 * n))
-
-
 PEAK_FP32_VECTOR = 157.3e12   # MI355X FP32 vector (packed) FLOP/s
 PEAK_FP64_VECTOR = 78.6e12
 PEAK_HBM = 8.0e12
+GROUP = 128
+
+
+def flops_per_point_eval(spinor, n, K=1, spline=False):
+    """Algorithmic FLOPs per point-eval with n Simpson nodes per segment and K
+    image tubes: scalar 5 + K(21 + 9n), spinor 5 + K(27 + 11n); a spline node
+    costs 5 more than a Gaussian node."""
+    per_node = (11 if spinor else 9) + (5 if spline else 0)
+    return 5 + K * ((27 if spinor else 21) + per_node * n)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------------ workloads
+def tuples_grid(geom_ids, rhos_per_geom, T_per_geom, cy_per_geom=None):
+    tg, tcx, tcy, tT = [], [], [], []
+    for g, rhos, Ts in zip(geom_ids, rhos_per_geom, T_per_geom):
+        cys = cy_per_geom[g] if cy_per_geom is not None else np.zeros(len(rhos))
+        for rho, cy in zip(rhos, cys):
+            for t in Ts:
+                tg.append(g); tcx.append(rho); tcy.append(cy); tT.append(t)
+    return np.array(tg), np.array(tcx), np.array(tcy), np.array(tT)
 
 
 def config_c2():
-    lams = [0.5, 1.0, 2.0]
-    Fs = [0.1, 1.0, 10.0]
-    geoms = [(F, lam * lam) for lam in lams for F in Fs]
-    nrho, nT = 64, 96
-    T = np.logspace(-3, 2, nT)
-    tg, tcx, tT = [], [], []
-    for g, (F, l2) in enumerate(geoms):
-        lam = np.sqrt(l2)
-        for rho in np.linspace(0.0, 4.0 * lam, nrho):
-            for t in T:
-                tg.append(g); tcx.append(rho); tT.append(t)
-    return dict(name="spinor_gaussian_5120x4096_9geom_64rho_96T", Nl=5120, N=4096, config_id=2, spinor=1,
-                nsub=2, geoms=geoms, nrho=nrho, nT=nT, T=T,
-                tup=(np.array(tg), np.array(tcx), np.zeros(len(tcx)), np.array(tT)))
+    lams, Fs = [0.5, 1.0, 2.0], [0.1, 1.0, 10.0]
+    geoms = [dict(kind=3, F=F, l2=lam * lam) for lam in lams for F in Fs]
+    T = np.logspace(-3, 2, 96)
+    rhos = [np.linspace(0.0, 4.0 * np.sqrt(g["l2"]), 64) for g in geoms]
+    return dict(name="spinor_gaussian_5120x4096_9geom_64rho_96T", Nl=5120, N=4096, config_id=2, spinor=1, nsub=2,
+                geoms=geoms, rhos=rhos, Ts=[T] * len(geoms),
+                tup=tuples_grid(range(len(geoms)), rhos, [T] * len(geoms)))
 
 
-def config_c1():
-    geoms = [(1.0, 1.0)]
-    nrho, nT = 32, 64
-    T = np.logspace(-3, 2, nT)
-    tg, tcx, tT = [], [], []
-    for rho in np.linspace(0.0, 3.0, nrho):
-        for t in T:
-            tg.append(0); tcx.append(rho); tT.append(t)
-    return dict(name="scalar_gaussian_1024x1024_32rho_64T", Nl=1024, N=1024, config_id=1, spinor=0, nsub=2,
-                geoms=geoms, nrho=nrho, nT=nT, T=T,
-                tup=(np.array(tg), np.array(tcx), np.zeros(len(tcx)), np.array(tT)))
+def gaussian_spline_table(wfl, l2=1.0, n=512, rho_max=20.0):
+    """Knots at rho = (i+1) rho_max/n (the layout of periodic.c:129-151) and
+    the clamped cubic spline of the Gaussian tube's f(rho^2)."""
+    rho = (np.arange(n) + 1) * (rho_max / n)
+    knots = (rho * rho).astype(np.float32)
+    fli = -np.expm1(-knots.astype(np.float64) / l2)
+    return knots, wfl.spline_build(fli, knots)
 
 
-def leff_table(wfl, cfg, gsum, group, policy):
-    """L_eff per (geometry, rho) and per group, from group sums."""
-    nrho, nT = cfg["nrho"], cfg["nT"]
-    ng = gsum.shape[1]
-    counts = np.full(ng, group, np.int32)
-    tg, tcx = cfg["tup"][0], cfg["tup"][1]
-    out = np.zeros((len(cfg["geoms"]), nrho, ng))
-    for g, (F, l2) in enumerate(cfg["geoms"]):
-        geom = wfl.Geometry(kind=wfl.GAUSSIAN, F=F, l2=l2)
-        for r in range(nrho):
-            base = (g * nrho + r) * nT
-            out[g, r] = wfl.leff_groups(geom, cfg["spinor"], cfg["T"], tcx[base], 0.0, gsum[base:base + nT],
-                                        counts, policy)
+def config_c1(wfl):
+    T = np.logspace(-3, 2, 64)
+    rhos = [np.linspace(0.0, 3.0, 32)]
+    geoms = [dict(kind=5, F=1.0, l2=1.0, a=1.0, spline_table=0)]
+    return dict(name="scalar_spline512_1024x1024_32rho_64T", Nl=1024, N=1024, config_id=1, spinor=0, nsub=2,
+                geoms=geoms, rhos=rhos, Ts=[T], splines=[gaussian_spline_table(wfl)],
+                tup=tuples_grid([0], rhos, [T]))
+
+
+def config_c3(scale, spinor):
+    Nl = max(GROUP, int(10240 * scale) // GROUP * GROUP)
+    ls = [0.1, 0.3, 1.0, 3.0, 10.0]
+    T = np.logspace(-3, 2, 64)
+    geoms = [dict(kind=3, F=1.0, l2=1.0, lattice=l, images=2) for l in ls]
+    cxs, cys = [], []
+    for l in ls:
+        c = (np.arange(32) + 0.5) * (l / 32.0)
+        X, Y = np.meshgrid(c, c, indexing="ij")
+        cxs.append(X.ravel()); cys.append(Y.ravel())
+    geoms.append(dict(kind=3, F=1.0, l2=1.0))            # the isolated tube, for the interaction energy
+    cxs.append(np.linspace(0.0, 6.0, 49)); cys.append(np.zeros(49))
+    tup = tuples_grid(range(len(geoms)), cxs, [T] * len(geoms), cy_per_geom=cys)
+    return dict(name=f"{'spinor' if spinor else 'scalar'}_lattice5x5_{Nl}x2048_32x32cell_64T", Nl=Nl, N=2048,
+                config_id=3, spinor=spinor, nsub=2, geoms=geoms, rhos=cxs, cys=cys, Ts=[T] * len(geoms),
+                lattices=ls, tup=tup)
+
+
+def config_c4(pool_loops=8192):
+    rng = np.random.default_rng(42)
+    geoms, rhos, Ts, nloops = [], [], [], []
+    for _ in range(48):
+        lam = rng.uniform(0.3, 3.0)
+        F = float(np.exp(rng.uniform(np.log(0.01), np.log(10.0))))
+        iso = rng.uniform() < 0.25
+        lat = 0.0 if iso else rng.uniform(0.5, 10.0) * lam
+        nr, nt = int(rng.integers(8, 129)), int(rng.integers(16, 97))
+        nl = int(rng.integers(1024, 8193))
+        geoms.append(dict(kind=3, F=F, l2=lam * lam, lattice=lat, images=0 if iso else 1))
+        rmax = 4.0 * lam if iso else min(4.0 * lam, 0.5 * lat)
+        rhos.append(np.linspace(0.0, rmax, nr))
+        Ts.append(np.logspace(-3, 2, nt))
+        nloops.append(min(nl, pool_loops))
+    return dict(name="mixed_48geom_uneven_pool8192x2048", Nl=pool_loops, N=2048, config_id=4, spinor=1, nsub=2,
+                geoms=geoms, rhos=rhos, Ts=Ts, geom_nloops=np.array(nloops),
+                tup=tuples_grid(range(48), rhos, Ts))
+
+
+def config_c5(scale):
+    Nl = max(GROUP, int(65536 * scale) // GROUP * GROUP)
+    T = np.logspace(-4, 2, 128)
+    rhos = [np.linspace(0.0, 5.0, 256)]
+    geoms = [dict(kind=3, F=1.0, l2=1.0, lattice=2.0, images=1)]
+    return dict(name=f"spinor_lattice3x3_{Nl}x8192_256rho_128T", Nl=Nl, N=8192, config_id=5, spinor=1, nsub=2,
+                geoms=geoms, rhos=rhos, Ts=[T], tup=tuples_grid([0], rhos, [T]))
+
+
+# ------------------------------------------------------------------ helpers
+def make_geoms(wfl, cfg):
+    return [wfl.Geometry(**g) for g in cfg["geoms"]]
+
+
+def point_evals(cfg):
+    tg = cfg["tup"][0]
+    nl = cfg.get("geom_nloops")
+    if nl is None:
+        return cfg["Nl"] * cfg["N"] * len(tg)
+    return int(sum(int(nl[g]) for g in tg)) * cfg["N"]
+
+
+def algorithmic_flops(cfg):
+    tg = cfg["tup"][0]
+    nl = cfg.get("geom_nloops")
+    tot = 0.0
+    for g, gd in enumerate(cfg["geoms"]):
+        K = (2 * gd.get("images", 0) + 1) ** 2 if gd.get("lattice", 0.0) > 0 else 1
+        cnt = int(np.sum(tg == g)) * (int(nl[g]) if nl is not None else cfg["Nl"]) * cfg["N"]
+        tot += cnt * flops_per_point_eval(cfg["spinor"], cfg["nsub"], K, spline=gd["kind"] == 5)
+    return tot
+
+
+def leff_table(wfl, cfg, gsum, policy):
+    """L_eff per group for every (geometry, x_cm) slice of the tuple list."""
+    out = []
+    tg, tcx, tcy, _ = cfg["tup"]
+    nl = cfg.get("geom_nloops")
+    base = 0
+    for g, gd in enumerate(cfg["geoms"]):
+        geom = wfl.Geometry(**gd)
+        T = cfg["Ts"][g]
+        nloops = int(nl[g]) if nl is not None else cfg["Nl"]
+        ngrp = (nloops + GROUP - 1) // GROUP
+        counts = np.array([min(GROUP, nloops - q * GROUP) for q in range(ngrp)], np.int32)
+        rows = []
+        for _ in range(len(cfg["rhos"][g])):
+            sl = gsum[base:base + len(T), :ngrp]
+            rows.append(wfl.leff_groups(geom, cfg["spinor"], T, tcx[base], tcy[base], sl, counts, policy))
+            base += len(T)
+        out.append(np.array(rows))
     return out
 
 
-def cpu_baseline(cfg, x, y, seconds=12.0):
-    """FP64 C oracle (OpenMP over loops) on a bounded sample of the workload."""
+def accuracy(wfl, cfg, gs_mixed, gs_64):
+    """Mixed against FP64 on the same loops.  Per geometry: the L_eff(x_cm)
+    error relative to that geometry's largest |L_eff|, the relative error of
+    the action (the sum over x_cm), the largest error in units of the
+    standard error of the mean over loop groups, and the median SEM/|L_eff|
+    (above ~0.1 the cell is dominated by Monte-Carlo noise: the sign problem
+    of cos(Phi) cosh(S) at strong fields)."""
+    res = {}
+    for policy, tag in ((1, "tfunc_policy"), (0, "worldline_all_T")):
+        Lm = leff_table(wfl, cfg, gs_mixed, policy)
+        Ld = leff_table(wfl, cfg, gs_64, policy)
+        per = []
+        for g in range(len(Lm)):
+            mm, md = Lm[g].mean(-1), Ld[g].mean(-1)
+            ng = Ld[g].shape[-1]
+            sem = Ld[g].std(-1, ddof=1) / np.sqrt(ng) if ng > 1 else None
+            scale = max(np.max(np.abs(md)), 1e-300)
+            per.append({"leff_max_err_over_max_leff": float(np.max(np.abs(mm - md)) / scale),
+                        "action_rel_err": float(abs(mm.sum() - md.sum()) / max(abs(md.sum()), 1e-300)),
+                        "max_err_over_sem": None if sem is None else
+                        float(np.max(np.abs(mm - md) / np.maximum(sem, 1e-300))),
+                        "median_sem_over_leff": None if sem is None else
+                        float(np.median(sem / np.maximum(np.abs(md), 1e-300)))})
+        res[tag] = per
+    c = np.array([8 / 3, -2.0, 1 / 3])
+    w_m = 1.0 + gs_mixed.sum(axis=1) @ c / cfg["Nl"]
+    w_d = 1.0 + gs_64.sum(axis=1) @ c / cfg["Nl"]
+    res["W_max_rel_err"] = float(np.max(np.abs(w_m - w_d) / np.abs(w_d)))
+    well = [p["action_rel_err"] for p in res["tfunc_policy"]
+            if p["median_sem_over_leff"] is not None and p["median_sem_over_leff"] < 0.1]
+    res["action_rel_err_max_well_conditioned"] = float(max(well)) if well else None
+    res["max_err_over_sem_all"] = float(max((p["max_err_over_sem"] or 0.0) for p in res["tfunc_policy"]))
+    return res
+
+
+def oracle_geoms(cfg):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
+    geoms = []
+    for gd in cfg["geoms"]:
+        kw = dict(F=gd["F"], l2=gd["l2"], lattice=gd.get("lattice", 0.0), images=gd.get("images", 0))
+        if gd["kind"] == 5:
+            kn, cf = cfg["splines"][gd["spline_table"]]
+            kw.update(tubedist=gd.get("a", 1.0), knots=kn, coefs=cf)
+        geoms.append(orc.Geom(gd["kind"], **kw))
+    return orc, geoms
+
+
+def cpu_baseline(cfg, x, y, mode_id, seconds=12.0, full=False):
+    """FP64 C oracle (OpenMP over loops) on a bounded sample of the workload,
+    or on the whole of it with full=True (then also returns its group sums)."""
+    orc, geoms = oracle_geoms(cfg)
     cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
     tg, tcx, tcy, tT = cfg["tup"]
-    geoms = [orc.Geom(orc.GAUSSIAN, F=F, l2=l2) for F, l2 in cfg["geoms"]]
-    nl = 64
+    if full:
+        t0 = time.perf_counter()
+        gs, _ = orc.eval_batch(geoms, mode_id, cfg["nsub"], cfg["spinor"], x, y, GROUP, tg, tcx, tcy, tT,
+                               perloop=False, nthreads=cores)
+        dt = time.perf_counter() - t0
+        return {"value": point_evals(cfg) / dt, "unit": "point-evals/s", "cores": cores, "kind": "port",
+                "sample": f"the whole workload, FP64 C oracle, {dt:.1f} s"}, gs
+    nl = min(64, x.shape[0])
     stride = max(1, len(tg) // 64)
     idx = np.arange(0, len(tg), stride)
     t0 = time.perf_counter()
-    orc.eval_batch(geoms, orc.MODE_PHYSICAL, cfg["nsub"], cfg["spinor"], x[:nl], y[:nl], nl, tg[idx], tcx[idx],
-                   tcy[idx], tT[idx], perloop=False, nthreads=cores)
+    orc.eval_batch(geoms, mode_id, cfg["nsub"], cfg["spinor"], x[:nl], y[:nl], nl, tg[idx], tcx[idx], tcy[idx],
+                   tT[idx], perloop=False, nthreads=cores)
     dt = time.perf_counter() - t0
-    # scale the tuple sample to roughly `seconds` of CPU work
     scale = max(1, int(seconds / max(dt, 1e-3)))
     stride2 = max(1, stride // scale)
     idx = np.arange(0, len(tg), stride2)
     t0 = time.perf_counter()
-    orc.eval_batch(geoms, orc.MODE_PHYSICAL, cfg["nsub"], cfg["spinor"], x[:nl], y[:nl], nl, tg[idx], tcx[idx],
-                   tcy[idx], tT[idx], perloop=False, nthreads=cores)
+    orc.eval_batch(geoms, mode_id, cfg["nsub"], cfg["spinor"], x[:nl], y[:nl], nl, tg[idx], tcx[idx], tcy[idx],
+                   tT[idx], perloop=False, nthreads=cores)
     dt = time.perf_counter() - t0
     pe = nl * cfg["N"] * len(idx)
     return {"value": pe / dt, "unit": "point-evals/s", "cores": cores, "kind": "port",
             "sample": f"first {nl} loops x {cfg['N']} points x every {stride2}th of {len(tg)} tuples "
-                      f"({len(idx)} tuples), FP64 C oracle, {dt:.1f} s"}
+                      f"({len(idx)} tuples), FP64 C oracle, {dt:.1f} s"}, None
+
+
+class Runner:
+    def __init__(self, wfl, cfg, loops, mode, mapping):
+        self.wfl, self.cfg, self.loops, self.mode, self.mapping = wfl, cfg, loops, mode, mapping
+
+    def plan(self, prec, mode=None):
+        cfg, wfl = self.cfg, self.wfl
+        mapping = self.mapping if prec == wfl.MIXED else max(self.mapping, 1)   # FP64: no tuning runs
+        b = wfl.Batch(make_geoms(wfl, cfg), *cfg["tup"], mode=self.mode if mode is None else mode, precision=prec,
+                      nsub=cfg["nsub"], spinor=cfg["spinor"], group=GROUP, splines=cfg.get("splines", []),
+                      geom_nloops=cfg.get("geom_nloops"), mapping=mapping)
+        return wfl.Plan(b, self.loops)
+
+    def timed(self, plan, steps, warmup, label):
+        wfl = self.wfl
+        for _ in range(warmup):
+            plan.run()
+        wfl.check(wfl.lib.wfl_device_sync())
+        kms = []
+        t0 = time.perf_counter()
+        for s in range(steps):
+            kms.append(plan.run_timed())
+            log(f"{label} step {s}: kernel {kms[-1]:.1f} ms")
+        wfl.check(wfl.lib.wfl_device_sync())
+        return (time.perf_counter() - t0) / steps, float(np.mean(kms)) / 1e3
+
+
+def load_loops(wfl, cfg):
+    t0 = time.perf_counter()
+    x, y = wfl.gen_loops(cfg["config_id"], cfg["Nl"], cfg["N"])
+    loops = wfl.Loops(x, y)
+    log(f"loops {cfg['Nl']}x{cfg['N']} generated+uploaded in {time.perf_counter() - t0:.1f}s")
+    return x, y, loops
+
+
+# ------------------------------------------------------------------ runs
+def interaction_energy(wfl, cfg, gsum):
+    """Per-tube action of the lattice cell minus the isolated tube's action.
+    Cell: midpoint rule over the 32 x 32 x_cm grid, (l^2/1024) sum L_eff.
+    Isolated: 2 pi int rho L_eff(rho) drho (Simpson on the 49-point rho grid)."""
+    L = leff_table(wfl, cfg, gsum, 1)
+    iso = L[-1]                                  # [rho, group]
+    rho = cfg["rhos"][-1]
+    h = rho[1] - rho[0]
+    w = np.full(len(rho), 2.0)
+    w[1::2] = 4.0
+    w[0] = w[-1] = 1.0
+    s_iso_groups = h / 3 * (w[:, None] * 2 * np.pi * rho[:, None] * iso).sum(0)
+    res = []
+    for i, l in enumerate(cfg["lattices"]):
+        e = L[i].sum(0) * (l * l / 1024.0) - s_iso_groups
+        res.append({"l": l, "E_int": float(e.mean()),
+                    "E_int_sem": float(e.std(ddof=1) / np.sqrt(len(e))) if len(e) > 1 else None})
+    return {"isolated_action": float(s_iso_groups.mean()), "per_tube": res}
+
+
+def run_config(wfl, args, mode, cfg):
+    x, y, loops = load_loops(wfl, cfg)
+    r = Runner(wfl, cfg, loops, mode, args.mapping)
+    pe = point_evals(cfg)
+    out = {"config": {"workload": cfg["name"], "loops": cfg["Nl"], "points": cfg["N"],
+                      "geometries": len(cfg["geoms"]), "tuples": int(len(cfg["tup"][0])), "simpson_n": cfg["nsub"],
+                      "segment_mode": args.mode, "point_evals_per_step": pe},
+           "unit": "point-evals/s", "n_gpus": 1, "data": "synthetic (seeded Philox unit loops)"}
+    mixed = r.plan(wfl.MIXED)
+    if args.mapping == 0:
+        t0 = time.perf_counter()
+        out["autotune_choice"] = mixed.autotune()
+        log(f"autotune {out['autotune_choice']} in {time.perf_counter() - t0:.1f}s")
+    out["launch_config"] = dict(zip(("mapping", "persistent", "wavefronts_per_block"), mixed.config()))
+    ms, km = r.timed(mixed, args.steps, args.warmup, "mixed")
+    if args.config == "c4":
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            mixed.graph_run()
+        wfl.check(wfl.lib.wfl_device_sync())
+        out["graph_ms_per_step"] = (time.perf_counter() - t0) / args.steps * 1e3
+    gs_m, _ = mixed.read()
+    out["nonfinite_loops_mixed"] = mixed.nonfinite()
+    mixed.close()
+    fl = algorithmic_flops(cfg)
+    out.update({"value": pe / ms, "ms_per_step": ms * 1e3, "kernel_ms": km * 1e3,
+                "valu_tflops_algorithmic": fl / km / 1e12, "frac_fp32_vector_peak": fl / km / PEAK_FP32_VECTOR})
+    gs_d = None
+    if not args.no_fp64:
+        fp64 = r.plan(wfl.FP64)
+        ms64, k64 = r.timed(fp64, args.fp64_steps, 0, "fp64")
+        gs_d, _ = fp64.read()
+        fp64.close()
+        out["fp64_value"] = pe / ms64
+        out["mixed_over_fp64"] = ms64 / ms
+        out["accuracy_mixed_vs_fp64"] = accuracy(wfl, cfg, gs_m, gs_d)
+    loops.close()
+    if "lattices" in cfg:
+        out["interaction_energy"] = interaction_energy(wfl, cfg, gs_d if gs_d is not None else gs_m)
+    if not args.no_cpu:
+        log("cpu baseline...")
+        full = args.config == "c1"
+        cb, gs_cpu = cpu_baseline(cfg, x, y, mode, full=full)
+        out["cpu_baseline"] = cb
+        if gs_cpu is not None and gs_d is not None:
+            out["fp64_gpu_vs_cpu_oracle_max_rel_err"] = float(np.max(np.abs(gs_d - gs_cpu)) /
+                                                             max(np.max(np.abs(gs_cpu)), 1e-300))
+    print(json.dumps(out))
+
+
+def bench_c2(wfl, args, mode):
+    cfg = config_c2()
+    x, y, loops = load_loops(wfl, cfg)
+    r = Runner(wfl, cfg, loops, mode, args.mapping)
+    pe_step = point_evals(cfg)
+    mixed = r.plan(wfl.MIXED)
+    mapping_used = mixed.mapping
+    ms_step, k_mixed = r.timed(mixed, args.steps, args.warmup, "mixed")
+    gs_mixed, _ = mixed.read()
+    nonfinite = mixed.nonfinite()
+    mixed.close()
+    result = {
+        "metric": "worldline point-evals/s, mixed vs FP64 on 1 GPU, at <=1e-6 rel. err in L_eff",
+        "value": pe_step / ms_step,
+        "unit": "point-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 segment math, fp64 sums",
+        "data": "synthetic (seeded Philox unit loops)",
+        "config": {"workload": cfg["name"], "loops": cfg["Nl"], "points": cfg["N"], "geometries": len(cfg["geoms"]),
+                   "tuples": int(len(cfg["tup"][0])), "simpson_n": cfg["nsub"], "segment_mode": args.mode,
+                   "group": GROUP, "point_evals_per_step": pe_step, "parallelism": "single GPU",
+                   "mapping": mapping_used},
+    }
+    fl = flops_per_point_eval(cfg["spinor"], cfg["nsub"])
+    achieved = fl * pe_step / k_mixed
+    result["roofline"] = {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32_VECTOR / 1e12,
+                          "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_VECTOR, "traffic": None,
+                          "kernel": "k_tuple<float,GAUSSIAN,spinor,physical,n=2>",
+                          "kernel_ms": k_mixed * 1e3, "flops_per_point_eval": fl}
+    tr = os.path.join(ROOT, "profiles", f"traffic_{cfg['name']}.json")
+    if os.path.exists(tr):
+        with open(tr) as fh:
+            result["roofline"]["traffic"] = json.load(fh).get("hbm_bytes_per_launch")
+    result["nonfinite_loops"] = nonfinite
+    if not args.no_fp64:
+        fp64 = r.plan(wfl.FP64)
+        ms64, k64 = r.timed(fp64, args.fp64_steps, 0, "fp64")
+        gs64, _ = fp64.read()
+        fp64.close()
+        result["fp64_value"] = pe_step / ms64
+        result["mixed_over_fp64"] = result["value"] / (pe_step / ms64)
+        result["fp64_kernel_ms"] = k64 * 1e3
+        result["accuracy_mixed_vs_fp64"] = accuracy(wfl, cfg, gs_mixed, gs64)
+    if not args.no_ref_mode and mode == wfl.MODE_PHYSICAL:
+        ref = r.plan(wfl.MIXED, wfl.MODE_REFERENCE)
+        msr, _ = r.timed(ref, 1, 0, "mixed-reference-mode")
+        ref.close()
+        result["reference_mode_mixed_value"] = pe_step / msr
+    loops.close()
+    if not args.no_cpu:
+        log("cpu baseline...")
+        result["cpu_baseline"], _ = cpu_baseline(cfg, x, y, mode)
+    print(json.dumps(result))
 
 
 def main():
@@ -116,8 +436,12 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--fp64-steps", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--scale", type=float, default=1.0, help="fraction of the loop ensemble (c3, c5)")
+    ap.add_argument("--species", default="both", choices=["scalar", "spinor", "both"], help="c3")
     ap.add_argument("--mode", default="physical", choices=["physical", "reference"])
+    ap.add_argument("--mapping", type=int, default=-1,
+                    help="0 autotune, 1 lane/tuple, 2 wave/loop, 3 lane/tuple split (default: 1 for c2, 0 otherwise)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-ref-mode", action="store_true")
@@ -142,113 +466,23 @@ def main():
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 (the multi-GPU sharding "
                                    "is described, not built)"}))
         sys.exit(2)
+    if args.mapping < 0:
+        args.mapping = 1 if args.config == "c2" else 0
 
     import __graft_entry__ as ge
 
     wfl = ge._import_pkg()
     if wfl.device_count() < 1:
         raise SystemExit("bench: no HIP device")
-    cfg = config_c2() if args.config == "c2" else config_c1()
     mode = wfl.MODE_PHYSICAL if args.mode == "physical" else wfl.MODE_REFERENCE
-    t0 = time.perf_counter()
-    x, y = wfl.gen_loops(cfg["config_id"], cfg["Nl"], cfg["N"])
-    loops = wfl.Loops(x, y)
-    print(f"[bench] loops {cfg['Nl']}x{cfg['N']} generated+uploaded in {time.perf_counter() - t0:.1f}s",
-          file=sys.stderr, flush=True)
-    group = 128
-    geoms = [wfl.Geometry(kind=wfl.GAUSSIAN, F=F, l2=l2) for F, l2 in cfg["geoms"]]
-    tg, tcx, tcy, tT = cfg["tup"]
-    ntup = len(tg)
-    pe_step = cfg["Nl"] * cfg["N"] * ntup
-
-    def make(prec, m=mode):
-        return wfl.Plan(wfl.Batch(geoms, tg, tcx, tcy, tT, mode=m, precision=prec, nsub=cfg["nsub"],
-                                  spinor=cfg["spinor"], group=group, mapping=1), loops)
-
-    def timed(plan, steps, warmup, label):
-        for _ in range(warmup):
-            plan.run()
-        wfl.check(wfl.lib.wfl_device_sync())
-        kms = []
-        t0 = time.perf_counter()
-        for s in range(steps):
-            kms.append(plan.run_timed())
-            print(f"[bench] {label} step {s}: kernel {kms[-1]:.1f} ms", file=sys.stderr, flush=True)
-        wfl.check(wfl.lib.wfl_device_sync())
-        wall = time.perf_counter() - t0
-        return wall / steps, float(np.mean(kms)) / 1e3
-
-    mixed = make(wfl.MIXED)
-    ms_step, k_mixed = timed(mixed, args.steps, args.warmup, "mixed")
-    gs_mixed, _ = mixed.read()
-    nonfinite = mixed.nonfinite()
-    mixed.close()
-    value = pe_step / ms_step
-    result = {
-        "metric": "worldline point-evals/s, mixed vs FP64 on 1 GPU, at <=1e-6 rel. err in L_eff",
-        "value": value,
-        "unit": "point-evals/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32 segment math, fp64 sums",
-        "data": "synthetic (seeded Philox unit loops)",
-        "config": {"workload": cfg["name"], "loops": cfg["Nl"], "points": cfg["N"], "geometries": len(geoms),
-                   "tuples": ntup, "simpson_n": cfg["nsub"], "segment_mode": args.mode, "group": group,
-                   "point_evals_per_step": pe_step, "parallelism": "single GPU"},
-    }
-    fl = flops_per_point_eval(cfg["spinor"], cfg["nsub"])
-    achieved = fl * pe_step / k_mixed
-    result["roofline"] = {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32_VECTOR / 1e12,
-                          "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_VECTOR, "traffic": None,
-                          "kernel": "k_tuple<float,GAUSSIAN,spinor,physical,n=2>",
-                          "kernel_ms": k_mixed * 1e3, "flops_per_point_eval": fl}
-    tr = os.path.join(ROOT, "profiles", f"traffic_{cfg['name']}.json")
-    if os.path.exists(tr):
-        with open(tr) as fh:
-            result["roofline"]["traffic"] = json.load(fh).get("hbm_bytes_per_launch")
-    result["nonfinite_loops"] = nonfinite
-
-    if not args.no_fp64:
-        fp64 = make(wfl.FP64)
-        ms64, k64 = timed(fp64, args.fp64_steps, 0, "fp64")
-        gs64, _ = fp64.read()
-        fp64.close()
-        result["fp64_value"] = pe_step / ms64
-        result["mixed_over_fp64"] = value / (pe_step / ms64)
-        result["fp64_kernel_ms"] = k64 * 1e3
-        # accuracy of mixed against FP64 on the full workload
-        ev_m = gs_mixed.sum(axis=1)
-        ev_d = gs64.sum(axis=1)
-        w_m = 1.0 + (8 / 3 * ev_m[:, 0] - 2 * ev_m[:, 1] + 1 / 3 * ev_m[:, 2]) / cfg["Nl"]
-        w_d = 1.0 + (8 / 3 * ev_d[:, 0] - 2 * ev_d[:, 1] + 1 / 3 * ev_d[:, 2]) / cfg["Nl"]
-        acc = {"W_max_rel_err": float(np.max(np.abs(w_m - w_d) / np.abs(w_d)))}
-        for policy, tag in ((1, "tfunc_policy"), (0, "worldline_all_T")):
-            Lm = leff_table(wfl, cfg, gs_mixed, group, policy)
-            Ld = leff_table(wfl, cfg, gs64, group, policy)
-            mean_m, mean_d = Lm.mean(-1), Ld.mean(-1)
-            sem_d = Ld.std(-1, ddof=1) / np.sqrt(Ld.shape[-1])
-            rel = np.abs(mean_m - mean_d) / np.maximum(np.abs(mean_d), 1e-300)
-            acc[f"Leff_{tag}_max_rel_err"] = float(rel.max())
-            acc[f"Leff_{tag}_median_rel_err"] = float(np.median(rel))
-            acc[f"Leff_{tag}_max_err_over_SEM"] = float(np.max(np.abs(mean_m - mean_d) / np.maximum(sem_d, 1e-300)))
-            acc[f"action_{tag}_rel_err"] = [float(abs(Lm[g].mean(-1).sum() - Ld[g].mean(-1).sum()) /
-                                                  max(abs(Ld[g].mean(-1).sum()), 1e-300)) for g in range(len(geoms))]
-        result["accuracy_mixed_vs_fp64"] = acc
-    if not args.no_ref_mode and mode == wfl.MODE_PHYSICAL:
-        ref = make(wfl.MIXED, wfl.MODE_REFERENCE)
-        msr, kr = timed(ref, 1, 0, "mixed-reference-mode")
-        ref.close()
-        result["reference_mode_mixed_value"] = pe_step / msr
-    loops.close()
-    if not args.no_cpu:
-        print("[bench] cpu baseline...", file=sys.stderr, flush=True)
-        result["cpu_baseline"] = cpu_baseline(cfg, x, y)
-    print(json.dumps(result))
+    if args.config == "c2":
+        return bench_c2(wfl, args, mode)
+    if args.config == "c3":
+        for sp in {"scalar": [0], "spinor": [1], "both": [0, 1]}[args.species]:
+            run_config(wfl, args, mode, config_c3(args.scale, sp))
+        return
+    cfg = {"c1": lambda: config_c1(wfl), "c4": config_c4, "c5": lambda: config_c5(args.scale)}[args.config]()
+    run_config(wfl, args, mode, cfg)
 
 
 if __name__ == "__main__":

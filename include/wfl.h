@@ -95,7 +95,8 @@ typedef struct {
   const double *tup_T;  /* [ntup] proper time T (quantized: sqrt(T) to FP32) */
   const int *geom_nloops; /* [ngeom] loops used per geometry (NULL: all Nl) */
   int want_perloop;     /* also keep per-loop (scal[3], ferm[3]) FP64 outputs */
-  int mapping;          /* 0: autotune; 1: lane per tuple; 2: wave per loop */
+  int mapping;          /* 0: autotune; 1: lane per tuple; 2: wave per loop;
+                           3: lane per tuple with W-1 in a second kernel */
 } wfl_batch;
 
 typedef struct wfl_loops_s wfl_loops;   /* device-resident loop ensemble */
@@ -130,6 +131,15 @@ int wfl_plan_run(wfl_plan *p, void *stream);
 /* wfl_plan_run, then wait for it; *kernel_ms is the main Wilson-loop
  * kernel's duration measured with HIP events on `stream`. */
 int wfl_plan_run_timed(wfl_plan *p, void *stream, float *kernel_ms);
+/* Measured autotuning of the launch configuration (mapping, static or
+ * persistent grid, wavefronts per block); runs the plan once per candidate
+ * and keeps the fastest.  *chosen (may be NULL) = 100*mapping +
+ * 10*persistent + wavefronts per block.  Outputs afterwards hold a valid run. */
+int wfl_plan_autotune(wfl_plan *p, void *stream, int *chosen);
+/* Current launch configuration, and setting one explicitly
+ * (mapping 1..3, persistent 0/1, wavefronts per block 1, 2 or 4). */
+int wfl_plan_config(const wfl_plan *p, int *mapping, int *persistent, int *wpb);
+int wfl_plan_set_config(wfl_plan *p, int mapping, int persistent, int wpb);
 /* Capture one wfl_plan_run into a hipGraph and replay it (T-sweep reuse). */
 int wfl_plan_graph_run(wfl_plan *p, void *stream);
 /* Copy results to the host (synchronizes): gsum [ntup][ngroups][3] sums of
@@ -140,7 +150,7 @@ int wfl_plan_ngroups(const wfl_plan *p);
 const double *wfl_plan_gsum_device(const wfl_plan *p);
 /* Number of non-finite per-loop values met in the last run (diagnostic). */
 int wfl_plan_nonfinite(wfl_plan *p, long long *count);
-/* Mapping the autotuner chose (1: lane per tuple, 2: wave per loop). */
+/* Mapping the plan uses (1, 2 or 3 as in wfl_batch.mapping). */
 int wfl_plan_mapping(const wfl_plan *p);
 void wfl_plan_destroy(wfl_plan *p);
 /* create + run + read + destroy. */

@@ -48,7 +48,7 @@ GAUSS_T = [0.01, 0.1, 0.7, 3.0]
 GAUSS_RHO = [0.0, 0.6, 1.7]
 
 
-@pytest.mark.parametrize("mapping", [1, 2])
+@pytest.mark.parametrize("mapping", [1, 2, 3])
 @pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("spinor", [0, 1])
 def test_gaussian_fp64(gpu, orc, loops, mapping, mode, spinor):
@@ -61,7 +61,7 @@ def test_gaussian_fp64(gpu, orc, loops, mapping, mode, spinor):
     assert e1 < 1e-10 and e2 < 1e-10
 
 
-@pytest.mark.parametrize("mapping", [1, 2])
+@pytest.mark.parametrize("mapping", [1, 2, 3])
 @pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("spinor", [0, 1])
 def test_gaussian_mixed(gpu, orc, loops, mapping, mode, spinor):
@@ -83,7 +83,7 @@ def test_nsub50_and_multi_geometry(gpu, orc, loops, prec, tol, mode):
     og = [orc.Geom(orc.GAUSSIAN, F=F, l2=l2) for F, l2 in specs]
     parts = [tuples([0.0, 1.1], [0.05, 0.9], geom=g) for g in range(3)]
     tup = tuple(np.concatenate([p[k] for p in parts]) for k in range(4))
-    for mapping in (1, 2):
+    for mapping in (1, 2, 3):
         gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, mode, 50, 1, prec, mapping, tup)
         e1, e2 = relerr(pl, rpl), relerr(gs, rgs)
         print(f"  mapping {mapping}: perloop {e1:.2e} gsum {e2:.2e}")

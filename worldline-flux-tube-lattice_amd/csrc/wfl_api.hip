@@ -99,8 +99,13 @@ __global__ void k_aos_to_soa(const float4* __restrict__ aos, long long n, float*
   y[i] = v.y;
 }
 
-__global__ void k_zero_u64(unsigned long long* p) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0ull;
+// Clears the per-run counters (non-finite count, work-queue ticket).  A
+// kernel rather than a memset so that the run can be captured in a graph.
+__global__ void k_zero_counters(unsigned long long* nonfinite, int* queue) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    *nonfinite = 0ull;
+    if (queue) *queue = 0;
+  }
 }
 
 // Group sums of per-loop W - 1 (wave-per-loop mapping): one thread per
@@ -119,6 +124,33 @@ __global__ void k_group_sums(const double* __restrict__ wm1, const DevTuple* __r
   }
   double* o = gsum + idx * 3;
   o[0] = s0; o[1] = s1; o[2] = s2;
+}
+
+// Group sums from per-loop (Wsscal, Wsferm) (split finalize of the
+// lane-per-tuple mapping): one thread per (tuple, group), W - 1 in FP64 and
+// a sequential sum in loop order.
+__global__ void k_finalize_groups(const double* __restrict__ perloop, const DevTuple* __restrict__ tups, int ntup,
+                                  int Nl, int group, int ngroups, int mode, int spinor, double* __restrict__ gsum,
+                                  unsigned long long* __restrict__ nonfinite) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)ntup * ngroups) return;
+  int t = (int)(idx / ngroups), q = (int)(idx % ngroups);
+  int nl = tups[t].nloops;
+  int l0 = q * group, l1 = std::min(l0 + group, nl);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  unsigned long long bad = 0;
+  for (int l = l0; l < l1; ++l) {
+    const double* p = perloop + ((size_t)t * Nl + l) * 6;
+    double sc[3] = {p[0], p[1], p[2]}, fe[3] = {p[3], p[4], p[5]}, wm[3];
+    wminus1(mode, spinor != 0, sc, fe, wm);
+    if (!(finite_r(wm[0]) && finite_r(wm[1]) && finite_r(wm[2]))) ++bad;
+    s0 += wm[0]; s1 += wm[1]; s2 += wm[2];
+  }
+  if (bad) atomicAdd(nonfinite, bad);
+  if (l1 > l0) {
+    double* o = gsum + idx * 3;
+    o[0] = s0; o[1] = s1; o[2] = s2;
+  }
 }
 
 // ------------------------------------------------------ loop ensembles
@@ -329,6 +361,9 @@ struct wfl_plan_s {
   double* d_perloop = nullptr;
   double* d_wm1 = nullptr;
   unsigned long long* d_nonfinite = nullptr;
+  int* d_queue = nullptr;
+  int wpb = 4, persistent = 0, grid = 0;   // launch configuration (autotuner)
+  bool tuned = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipStream_t gstream = nullptr;
@@ -344,7 +379,7 @@ extern "C" void wfl_plan_destroy(wfl_plan* p) {
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
   void* bufs[] = {p->d_tups, p->d_geoms, p->d_lanes, p->d_tasks, p->d_knots, p->d_coefs,
-                  p->d_gsum, p->d_perloop, p->d_wm1, p->d_nonfinite};
+                  p->d_gsum, p->d_perloop, p->d_wm1, p->d_nonfinite, p->d_queue};
   for (void* b : bufs) if (b) (void)hipFree(b);
   delete p;
 }
@@ -370,7 +405,7 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
   if (b->ntup < 0 || (b->ntup > 0 && (!b->tup_geom || !b->tup_cx || !b->tup_cy || !b->tup_T))) {
     wfl_set_error("tuple arrays missing"); return WFL_ERR_ARG;
   }
-  if (b->mapping < 0 || b->mapping > 2) { wfl_set_error("bad mapping %d", b->mapping); return WFL_ERR_ARG; }
+  if (b->mapping < 0 || b->mapping > 3) { wfl_set_error("bad mapping %d", b->mapping); return WFL_ERR_ARG; }
   for (int g = 0; g < b->ngeom; ++g) { int rc = check_geom(b->geoms[g], b->nspline, b->splines); if (rc) return rc; }
   for (int t = 0; t < b->ntup; ++t) {
     if (b->tup_geom[t] < 0 || b->tup_geom[t] >= b->ngeom) { wfl_set_error("tuple %d: geometry index out of range", t); return WFL_ERR_ARG; }
@@ -447,13 +482,26 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
       int l0 = q * b->group, l1 = std::min(l0 + b->group, lg_nloops[lg]);
       if (l1 > l0) tasks.push_back(Task{lg, l0, l1 - l0, q});
     }
+  // longest tasks first (loops x image tubes), so the tail of the launch is short
+  std::vector<int> lg_images(nlg, 1);
+  for (int lg = 0; lg < nlg; ++lg) {
+    int t0 = lanes[(size_t)lg * 64];
+    const wfl_geometry& G = b->geoms[p->tups[t0].geom];
+    if (G.lattice > 0.0) lg_images[lg] = (2 * G.images + 1) * (2 * G.images + 1);
+  }
+  std::stable_sort(tasks.begin(), tasks.end(), [&](const Task& u, const Task& v) {
+    return (long long)u.nloop * lg_images[u.lanegroup] > (long long)v.nloop * lg_images[v.lanegroup];
+  });
   p->ntasks = (int)tasks.size();
   // mapping (autotune heuristic: lane-per-tuple needs enough lane groups)
   int mapping = b->mapping;
+  const double perloop_bytes = 48.0 * (double)std::max(b->ntup, 1) * (double)Nl;
   if (mapping == 0) {
     double util = b->ntup > 0 ? (double)b->ntup / (64.0 * std::max(nlg, 1)) : 0.0;
     mapping = (util >= 0.5 && p->ntasks >= 512) ? 1 : 2;
+    (void)perloop_bytes;   // mapping 3 measured no faster than 1 on the bench workload: opt-in only
   }
+  if (mapping == 3 && !p->prof_gauss) mapping = 1;
   p->mapping = mapping;
 
   TRY(loops_records(L, b->precision));
@@ -469,7 +517,7 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
     hipError_t e = hipMemset(p->d_gsum, 0, sizeof(double) * ng3);
     if (e != hipSuccess) { wfl_set_error("hipMemset: %s", hipGetErrorString(e)); wfl_plan_destroy(p); return WFL_ERR_DEVICE; }
   }
-  if (b->want_perloop) {
+  if (b->want_perloop || mapping == 3) {
     size_t n6 = (size_t)std::max(b->ntup, 1) * Nl * 6;
     TRY(upload<double>(&p->d_perloop, nullptr, n6));
     hipError_t e = hipMemset(p->d_perloop, 0, sizeof(double) * n6);
@@ -477,6 +525,13 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
   }
   if (mapping == 2) TRY(upload<double>(&p->d_wm1, nullptr, (size_t)std::max(b->ntup, 1) * Nl * 3));
   TRY(upload<unsigned long long>(&p->d_nonfinite, nullptr, 1));
+  TRY(upload<int>(&p->d_queue, nullptr, 1));
+  {
+    int dev = 0, ncu = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) ncu = 256;
+    p->grid = ncu * 32 / p->wpb;     // persistent grid: 32 wavefronts per CU
+  }
   {
     hipError_t e = hipMemset(p->d_nonfinite, 0, sizeof(unsigned long long));
     if (e != hipSuccess) { wfl_set_error("hipMemset: %s", hipGetErrorString(e)); wfl_plan_destroy(p); return WFL_ERR_DEVICE; }
@@ -494,8 +549,13 @@ static void fill_launch(const wfl_plan* p, WflLaunch& a, hipStream_t s) {
   a.lanes = p->d_lanes; a.tasks = p->d_tasks; a.ntasks = p->ntasks; a.ngroups = p->ngroups;
   a.knots = p->d_knots; a.coefs = p->d_coefs;
   a.prof_gauss = p->prof_gauss; a.spinor = p->b.spinor ? 1 : 0; a.mode = p->b.mode; a.nsub = p->b.nsub;
+  a.split = p->mapping == 3;
   a.gsum = p->d_gsum; a.perloop = p->d_perloop; a.wm1 = p->d_wm1;
   a.nonfinite = p->d_nonfinite;
+  a.wpb = p->wpb;
+  a.persistent = p->persistent;
+  a.grid = p->grid;
+  a.queue = p->d_queue;
   a.stream = s;
 }
 
@@ -503,15 +563,22 @@ extern "C" int wfl_plan_run(wfl_plan* p, void* stream) {
   if (!p) { wfl_set_error("wfl_plan_run: NULL plan"); return WFL_ERR_STATE; }
   if (p->b.ntup == 0) return WFL_OK;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, p->d_nonfinite);
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, p->d_nonfinite, p->d_queue);
   HIP_TRY(hipGetLastError());
   WflLaunch a;
   fill_launch(p, a, s);
   int e;
   if (p->timed) HIP_TRY(hipEventRecord(p->ev0, s));
-  if (p->mapping == 1) {
+  if (p->mapping == 1 || p->mapping == 3) {
     e = p->b.precision == WFL_MIXED ? wfl_launch_tuple_f32(&a) : wfl_launch_tuple_f64(&a);
     if (p->timed && e == 0) e = (int)hipEventRecord(p->ev1, s);
+    if (e == 0 && p->mapping == 3) {
+      long long n = (long long)p->b.ntup * p->ngroups;
+      hipLaunchKernelGGL(k_finalize_groups, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, p->d_perloop,
+                         p->d_tups, p->b.ntup, p->L->Nl, p->b.group, p->ngroups, p->b.mode, p->b.spinor,
+                         p->d_gsum, p->d_nonfinite);
+      e = (int)hipGetLastError();
+    }
   } else {
     e = p->b.precision == WFL_MIXED ? wfl_launch_loop_f32(&a) : wfl_launch_loop_f64(&a);
     if (p->timed && e == 0) e = (int)hipEventRecord(p->ev1, s);
@@ -538,6 +605,110 @@ extern "C" int wfl_plan_run_timed(wfl_plan* p, void* stream, float* kernel_ms) {
   HIP_TRY(hipEventSynchronize(p->ev1));
   HIP_TRY(hipEventElapsedTime(kernel_ms, p->ev0, p->ev1));
   return WFL_OK;
+}
+
+// ------------------------------------------------------ autotuner
+// Measured choice of the launch configuration: mapping (lane per tuple
+// inline / split finalize, wave per loop for small batches), static or
+// persistent grid, and wavefronts per block.  Every candidate run is a full,
+// valid evaluation; the fastest configuration is kept on the plan and
+// remembered for later plans of the same shape in this process.
+struct TuneKey {
+  int ntup, Nl, N, precision, spinor, mode, nsub, prof_gauss, ntasks;
+  bool operator==(const TuneKey& o) const { return memcmp(this, &o, sizeof o) == 0; }
+};
+struct TuneVal { int mapping, persistent, wpb; };
+static std::vector<std::pair<TuneKey, TuneVal>> g_tuned;
+
+static int plan_apply(wfl_plan* p, int mapping, int persistent, int wpb) {
+  if (mapping == 3 && !p->d_perloop) {
+    size_t n6 = (size_t)std::max(p->b.ntup, 1) * p->L->Nl * 6;
+    HIP_TRY(hipMalloc(&p->d_perloop, sizeof(double) * n6));
+  }
+  if (mapping == 2 && !p->d_wm1) HIP_TRY(hipMalloc(&p->d_wm1, sizeof(double) * (size_t)std::max(p->b.ntup, 1) * p->L->Nl * 3));
+  int dev = 0, ncu = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) ncu = 256;
+  p->mapping = mapping;
+  p->persistent = persistent;
+  p->wpb = wpb;
+  p->grid = ncu * 32 / wpb;
+  if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+  if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+  return WFL_OK;
+}
+
+extern "C" int wfl_plan_autotune(wfl_plan* p, void* stream, int* chosen) {
+  if (!p) { wfl_set_error("wfl_plan_autotune: NULL plan"); return WFL_ERR_STATE; }
+  TuneKey key;
+  memset(&key, 0, sizeof key);
+  key.ntup = p->b.ntup; key.Nl = p->L->Nl; key.N = p->L->N; key.precision = p->b.precision;
+  key.spinor = p->b.spinor; key.mode = p->b.mode; key.nsub = p->b.nsub; key.prof_gauss = p->prof_gauss;
+  key.ntasks = p->ntasks;
+  for (auto& kv : g_tuned)
+    if (kv.first == key) {
+      int rc = plan_apply(p, kv.second.mapping, kv.second.persistent, kv.second.wpb);
+      if (rc) return rc;
+      if (chosen) *chosen = kv.second.mapping * 100 + kv.second.persistent * 10 + kv.second.wpb;
+      p->tuned = true;
+      return WFL_OK;
+    }
+  std::vector<TuneVal> cand;
+  for (int pers = 0; pers <= 1; ++pers)
+    for (int wpb : {2, 4}) {
+      cand.push_back(TuneVal{1, pers, wpb});
+      const double perloop_bytes = 48.0 * (double)std::max(p->b.ntup, 1) * (double)p->L->Nl;
+      if (p->prof_gauss && perloop_bytes <= 24.0e9) cand.push_back(TuneVal{3, pers, wpb});
+    }
+  if (p->b.ntup < 2048) cand.push_back(TuneVal{2, 0, 4});
+  TuneVal best = cand[0];
+  float best_ms = 1e30f;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  int rc = WFL_OK;
+  for (const TuneVal& c : cand) {
+    rc = plan_apply(p, c.mapping, c.persistent, c.wpb);
+    if (rc) break;
+    float ms = 0.0f;
+    (void)hipEventRecord(e0, (hipStream_t)stream);
+    rc = wfl_plan_run(p, stream);                 // the whole run, every kernel of it
+    if (rc) break;
+    (void)hipEventRecord(e1, (hipStream_t)stream);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      wfl_set_error("wfl_plan_autotune: event timing failed");
+      rc = WFL_ERR_DEVICE;
+      break;
+    }
+    if (ms < best_ms) { best_ms = ms; best = c; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  rc = plan_apply(p, best.mapping, best.persistent, best.wpb);
+  if (rc) return rc;
+  rc = wfl_plan_run(p, stream);      // leave the plan's outputs from the chosen configuration
+  if (rc) return rc;
+  g_tuned.push_back({key, best});
+  p->tuned = true;
+  if (chosen) *chosen = best.mapping * 100 + best.persistent * 10 + best.wpb;
+  return WFL_OK;
+}
+
+extern "C" int wfl_plan_config(const wfl_plan* p, int* mapping, int* persistent, int* wpb) {
+  if (!p) { wfl_set_error("wfl_plan_config: NULL plan"); return WFL_ERR_ARG; }
+  if (mapping) *mapping = p->mapping;
+  if (persistent) *persistent = p->persistent;
+  if (wpb) *wpb = p->wpb;
+  return WFL_OK;
+}
+
+extern "C" int wfl_plan_set_config(wfl_plan* p, int mapping, int persistent, int wpb) {
+  if (!p || mapping < 1 || mapping > 3 || (persistent != 0 && persistent != 1) || (wpb != 1 && wpb != 2 && wpb != 4)) {
+    wfl_set_error("wfl_plan_set_config: bad configuration"); return WFL_ERR_ARG;
+  }
+  if (mapping == 3 && !p->prof_gauss) { wfl_set_error("wfl_plan_set_config: split finalize needs the Gaussian fast path"); return WFL_ERR_ARG; }
+  return plan_apply(p, mapping, persistent, wpb);
 }
 
 extern "C" int wfl_plan_graph_run(wfl_plan* p, void* stream) {
@@ -651,6 +822,7 @@ extern "C" int wfl_internal_ev_launch(const void* dev_aos, int Nl, int N, const 
   a.prof_gauss = g->kind == WFL_GAUSSIAN; a.spinor = fermion == 1; a.mode = mode; a.nsub = nsub;
   a.ws_scal = (float4*)ws_scal_d; a.ws_ferm = fermion == 1 ? (float4*)ws_ferm_d : nullptr;
   a.nonfinite = g_ev.d_nf;
+  a.wpb = 4;
   a.stream = 0;
   int e = precision == WFL_MIXED ? wfl_launch_loop_f32(&a) : wfl_launch_loop_f64(&a);
   if (e) { wfl_set_error("EV kernel launch failed: %s", hipGetErrorString((hipError_t)e)); return WFL_ERR_DEVICE; }

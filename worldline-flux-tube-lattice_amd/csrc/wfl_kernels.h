@@ -208,18 +208,17 @@ __device__ __forceinline__ void finalize_loop(const double P[3], const double S[
 }
 
 // ------------------------------------------------------ lane per tuple
-template <class R, int PROF, int SPINOR, int MODE, int NSUB>
-__global__ __launch_bounds__(256) void k_tuple(
-    const R* __restrict__ rec, int N, int Nl, const DevTuple* __restrict__ tups,
+// SPLIT = 1 writes each loop's (Wsscal, Wsferm) to perloop and leaves W - 1
+// and the group sums to k_finalize_groups: the FP64 trigonometry then does
+// not set this kernel's register allocation (more resident wavefronts).
+template <class R, int PROF, int SPINOR, int MODE, int NSUB, int SPLIT>
+__device__ __forceinline__ void tuple_task(
+    int task_id, int lane, const R* __restrict__ rec, int N, int Nl, const DevTuple* __restrict__ tups,
     const DevGeom* __restrict__ geoms, const int* __restrict__ lanes,
     const float* __restrict__ knots, const float4* __restrict__ coefs,
-    const Task* __restrict__ tasks, int ntasks, int ngroups, int nsub_rt,
+    const Task* __restrict__ tasks, int ngroups, int nsub_rt,
     double* __restrict__ gsum, double* __restrict__ perloop,
     unsigned long long* __restrict__ nonfinite) {
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int task_id = blockIdx.x * 4 + wv;
-  if (task_id >= ntasks) return;
-  const int lane = threadIdx.x & 63;
   const Task tk = tasks[task_id];
   const int lg = __builtin_amdgcn_readfirstlane(tk.lanegroup);
   const int loop0 = __builtin_amdgcn_readfirstlane(tk.loop0);
@@ -274,20 +273,54 @@ __global__ __launch_bounds__(256) void k_tuple(
         }
       }
     }
-    double sc[3], fe[3], wm[3];
+    double sc[3], fe[3];
     finalize_loop(P, S, gm.F, s_phase, tp.sd * tp.sd, spin_scale, n, N, SPINOR, sc, fe);
-    wminus1(MODE, SPINOR, sc, fe, wm);
-    bool ok = finite_r(wm[0]) && finite_r(wm[1]) && finite_r(wm[2]);
-    if (valid && !ok) atomicAdd(nonfinite, 1ull);
-    G[0] += wm[0]; G[1] += wm[1]; G[2] += wm[2];
+    if (!SPLIT) {
+      double wm[3];
+      wminus1(MODE, SPINOR, sc, fe, wm);
+      bool ok = finite_r(wm[0]) && finite_r(wm[1]) && finite_r(wm[2]);
+      if (valid && !ok) atomicAdd(nonfinite, 1ull);
+      G[0] += wm[0]; G[1] += wm[1]; G[2] += wm[2];
+    }
     if (perloop && valid) {
       double* o = perloop + ((size_t)tid * Nl + l) * 6;
       o[0] = sc[0]; o[1] = sc[1]; o[2] = sc[2]; o[3] = fe[0]; o[4] = fe[1]; o[5] = fe[2];
     }
   }
-  if (valid) {
+  if (!SPLIT && valid) {
     double* o = gsum + ((size_t)tid * ngroups + grp) * 3;
     o[0] = G[0]; o[1] = G[1]; o[2] = G[2];
+  }
+}
+
+// Next task of a persistent wavefront: one lane takes a ticket (a vector
+// atomic), the wavefront reads it from that lane.
+__device__ __forceinline__ int next_task(int* __restrict__ queue, int lane) {
+  int t = 0;
+  if (lane == 0) t = atomicAdd(queue, 1);
+  return __shfl(t, 0, 64);
+}
+
+// wpb wavefronts per block.  Static: wavefront w of the grid runs task w.
+// Persistent: a grid sized to the machine; each wavefront keeps taking the
+// next task from *queue (zeroed before the launch) until none is left, so
+// uneven tasks (lattice images, partial loop groups) balance themselves.
+template <class R, int PROF, int SPINOR, int MODE, int NSUB, int SPLIT>
+__global__ __launch_bounds__(256) void k_tuple(
+    const R* __restrict__ rec, int N, int Nl, const DevTuple* __restrict__ tups,
+    const DevGeom* __restrict__ geoms, const int* __restrict__ lanes,
+    const float* __restrict__ knots, const float4* __restrict__ coefs,
+    const Task* __restrict__ tasks, int ntasks, int ngroups, int nsub_rt,
+    double* __restrict__ gsum, double* __restrict__ perloop,
+    unsigned long long* __restrict__ nonfinite, int wpb, int persistent, int* __restrict__ queue) {
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  int task_id = persistent ? next_task(queue, lane) : (int)blockIdx.x * wpb + wv;
+  while (task_id < ntasks) {
+    tuple_task<R, PROF, SPINOR, MODE, NSUB, SPLIT>(task_id, lane, rec, N, Nl, tups, geoms, lanes, knots, coefs,
+                                                   tasks, ngroups, nsub_rt, gsum, perloop, nonfinite);
+    if (!persistent) break;
+    task_id = next_task(queue, lane);
   }
 }
 
@@ -303,9 +336,9 @@ __global__ __launch_bounds__(256) void k_loop(
     const float* __restrict__ knots, const float4* __restrict__ coefs, int nsub_rt,
     double* __restrict__ perloop, double* __restrict__ wm1,
     float4* __restrict__ ws_scal, float4* __restrict__ ws_ferm,
-    unsigned long long* __restrict__ nonfinite) {
+    unsigned long long* __restrict__ nonfinite, int wpb) {
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const long long w = (long long)blockIdx.x * 4 + wv;
+  const long long w = (long long)blockIdx.x * wpb + wv;
   if (w >= (long long)ntup * Nl) return;
   const int t = (int)(w / Nl), l = (int)(w % Nl);
   const int lane = threadIdx.x & 63;
@@ -383,7 +416,9 @@ struct WflLaunch {
   const wfl::DevTuple* tups; const wfl::DevGeom* geoms; int ntup;
   const int* lanes; const wfl::Task* tasks; int ntasks; int ngroups;
   const float* knots; const float4* coefs;
-  int prof_gauss, spinor, mode, nsub;
+  int prof_gauss, spinor, mode, nsub, split;
+  int wpb, persistent, grid;   // wavefronts per block; persistent grid size in blocks
+  int* queue;
   double* gsum; double* perloop; double* wm1;
   float4* ws_scal; float4* ws_ferm;
   unsigned long long* nonfinite;

@@ -87,7 +87,8 @@ template <> struct GP<double> {
 // so that a node exactly on the axis (where z x dw = 0) gives 0, not NaN.
 __device__ __forceinline__ float gauss_g(float u, double, float& e) {
   e = __builtin_amdgcn_exp2f(-u);
-  return (1.0f - e) * __builtin_amdgcn_rcpf(u);
+  const float r = __builtin_amdgcn_rcpf(u);
+  return fmaf(-e, r, r);          // (1 - e)/u in one FMA
 }
 // FP64, problem units: g(u) = -expm1(-u/l2)/u (limit 1/l2 at u = 0),
 // e = 1 + expm1(-u/l2).

@@ -78,7 +78,8 @@ EXPORTS = [
     "wfl_last_error", "wfl_device_count", "wfl_device_sync",
     "wfl_loops_create", "wfl_loops_from_aos_device", "wfl_loops_generate", "wfl_loops_shape",
     "wfl_loops_destroy", "wfl_gen_loops", "wfl_read_text_loops",
-    "wfl_plan_create", "wfl_plan_run", "wfl_plan_run_timed", "wfl_plan_graph_run", "wfl_plan_read", "wfl_plan_ngroups",
+    "wfl_plan_create", "wfl_plan_run", "wfl_plan_run_timed", "wfl_plan_autotune", "wfl_plan_config",
+    "wfl_plan_set_config", "wfl_plan_graph_run", "wfl_plan_read", "wfl_plan_ngroups",
     "wfl_plan_gsum_device", "wfl_plan_nonfinite", "wfl_plan_mapping", "wfl_plan_destroy",
     "wfl_eval_batch", "wfl_group_ev_minus1", "wfl_spline_build", "wfl_set_profile",
     "wfl_set_numerics", "wfl_ev_reset", "wfl_ev_launches", "wfl_logsimpson", "wfl_effective_T",
@@ -108,6 +109,9 @@ def _load():
         "wfl_plan_run": (I, [P, P]),
         "wfl_plan_graph_run": (I, [P, P]),
         "wfl_plan_run_timed": (I, [P, P, P]),
+        "wfl_plan_autotune": (I, [P, P, P]),
+        "wfl_plan_config": (I, [P, P, P, P]),
+        "wfl_plan_set_config": (I, [P, I, I, I]),
         "wfl_plan_read": (I, [P, P, P]),
         "wfl_plan_ngroups": (I, [P]),
         "wfl_plan_gsum_device": (P, [P]),
@@ -301,6 +305,22 @@ class Plan:
         ms = C.c_float(0.0)
         check(lib.wfl_plan_run_timed(self.h, stream, C.byref(ms)))
         return ms.value
+
+    def autotune(self, stream=None):
+        """Measure the candidate launch configurations and keep the fastest."""
+        c = C.c_int(0)
+        check(lib.wfl_plan_autotune(self.h, stream, C.byref(c)))
+        self.mapping = lib.wfl_plan_mapping(self.h)
+        return c.value
+
+    def config(self):
+        m, pe, w = C.c_int(), C.c_int(), C.c_int()
+        check(lib.wfl_plan_config(self.h, C.byref(m), C.byref(pe), C.byref(w)))
+        return m.value, pe.value, w.value
+
+    def set_config(self, mapping, persistent, wpb):
+        check(lib.wfl_plan_set_config(self.h, mapping, persistent, wpb))
+        self.mapping = mapping
 
     def graph_run(self, stream=None):
         check(lib.wfl_plan_graph_run(self.h, stream))

@@ -332,7 +332,7 @@ def run_config(wfl, args, mode, cfg):
         t0 = time.perf_counter()
         out["autotune_choice"] = mixed.autotune()
         log(f"autotune {out['autotune_choice']} in {time.perf_counter() - t0:.1f}s")
-    out["launch_config"] = dict(zip(("mapping", "persistent", "wavefronts_per_block"), mixed.config()))
+    out["launch_config"] = dict(zip(("mapping", "persistent", "wavefronts_per_block", "tasks_per_group"), mixed.config()))
     ms, km = r.timed(mixed, args.steps, args.warmup, "mixed")
     if args.config == "c4":
         t0 = time.perf_counter()
@@ -349,6 +349,7 @@ def run_config(wfl, args, mode, cfg):
     gs_d = None
     if not args.no_fp64:
         fp64 = r.plan(wfl.FP64)
+        fp64.set_config(*out["launch_config"].values())   # the same launch shape as the mixed run
         ms64, k64 = r.timed(fp64, args.fp64_steps, 0, "fp64")
         gs_d, _ = fp64.read()
         fp64.close()

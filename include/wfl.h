@@ -132,14 +132,18 @@ int wfl_plan_run(wfl_plan *p, void *stream);
  * kernel's duration measured with HIP events on `stream`. */
 int wfl_plan_run_timed(wfl_plan *p, void *stream, float *kernel_ms);
 /* Measured autotuning of the launch configuration (mapping, static or
- * persistent grid, wavefronts per block); runs the plan once per candidate
- * and keeps the fastest.  *chosen (may be NULL) = 100*mapping +
- * 10*persistent + wavefronts per block.  Outputs afterwards hold a valid run. */
+ * persistent grid, wavefronts per block, tasks per loop group); runs the plan
+ * once per candidate and keeps the fastest.  *chosen (may be NULL) =
+ * 1000*mapping + 100*persistent + 10*wavefronts per block + tasks per loop
+ * group.  Outputs afterwards hold a valid run. */
 int wfl_plan_autotune(wfl_plan *p, void *stream, int *chosen);
-/* Current launch configuration, and setting one explicitly
- * (mapping 1..3, persistent 0/1, wavefronts per block 1, 2 or 4). */
-int wfl_plan_config(const wfl_plan *p, int *mapping, int *persistent, int *wpb);
-int wfl_plan_set_config(wfl_plan *p, int mapping, int persistent, int wpb);
+/* Current launch configuration, and setting one explicitly: mapping 1..3,
+ * persistent 0/1, wavefronts per block 1, 2 or 4, and `chunks` tasks per loop
+ * group (1..min(64, group); lane-per-tuple mappings: each task sums
+ * ceil(group/chunks) loops and the chunk sums are added in order, so small
+ * batches still fill the device). */
+int wfl_plan_config(const wfl_plan *p, int *mapping, int *persistent, int *wpb, int *chunks);
+int wfl_plan_set_config(wfl_plan *p, int mapping, int persistent, int wpb, int chunks);
 /* Capture one wfl_plan_run into a hipGraph and replay it (T-sweep reuse). */
 int wfl_plan_graph_run(wfl_plan *p, void *stream);
 /* Copy results to the host (synchronizes): gsum [ntup][ngroups][3] sums of

@@ -211,10 +211,16 @@ double orc_interp(double rho2, const float *k, const float *c, int n) {  /* intE
   return 0.0;
 }
 
-/* Spline in the physical mode: f = a_0 u / u_0 below the first knot (the
- * linear continuation through f(0) = 0), f = f(last knot) beyond the last. */
+/* Spline in the physical mode (no counterpart in the reference, whose interp
+ * returns 0 outside the knots): at or below the first knot the straight line
+ * f = s u through f(0) = 0 with the FP32 slope s = fl32(a_0 / u_0) (a_0 when
+ * the first knot is at 0), so that the line is one more table interval;
+ * f = f(last knot) at or beyond the last knot. */
+double orc_spline_low_slope(const float *k, const float *c) {
+  return k[0] > 0.0f ? (double)(float)((double)c[0] / (double)k[0]) : (double)c[1];
+}
 static double orc_interp_phys(double rho2, const float *k, const float *c, int n) {
-  if (rho2 <= (double)k[0]) return (k[0] > 0.0f) ? (double)c[0] * rho2 / (double)k[0] : (double)c[0];
+  if (rho2 <= (double)k[0]) return orc_spline_low_slope(k, c) * rho2;
   if (rho2 >= (double)k[n - 1]) return (double)c[4 * (n - 1)];
   return orc_interp(rho2, k, c, n);
 }
@@ -247,7 +253,7 @@ static double orc_fp_mode(const orc_geom *g, int mode, double u) {
     case ORC_SPLINE:
       if (mode == ORC_MODE_PHYSICAL) {
         const float *k = g->rho2sp; int n = g->nspline;
-        if (u <= (double)k[0]) return k[0] > 0.0f ? (double)g->coefs[0] / (double)k[0] : (double)g->coefs[1];
+        if (u <= (double)k[0]) return orc_spline_low_slope(k, g->coefs);
         if (u >= (double)k[n - 1]) return 0.0;
         int lo = 0, hi = n - 1;
         while (hi - lo > 1) { int mid = (hi + lo) / 2; if (u >= (double)k[mid]) lo = mid; else hi = mid; }

@@ -55,6 +55,8 @@ double orc_f(const orc_geom *g, double rho2);
 double orc_fp(const orc_geom *g, double rho2);
 double orc_expint(double x);
 double orc_interp(double rho2, const float *rho2sp, const float *coefs, int n);
+/* slope of the physical-mode spline line below the first knot */
+double orc_spline_low_slope(const float *rho2sp, const float *coefs);
 
 /* Per-loop sub-loop sums: scal[3] = F*(x, y, z) and ferm[3] as WilsonLoop
  * writes them (intEV.cu:553-560); lattice images summed. */

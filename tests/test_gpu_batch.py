@@ -1,7 +1,8 @@
 """Batched uneven workloads on the GPU: several geometries (isolated and
 lattice) with their own rho/T grids and loop counts in one launch, every
 launch configuration (lane per tuple inline / split, static / persistent
-grid, 1-4 wavefronts per block), the measured autotuner, and hipGraph replay.
+grid, 1-4 wavefronts per block, 1-4 tasks per loop group), the measured
+autotuner, and hipGraph replay.
 Checked against the FP64 oracle (per-loop W - 1 summed per group)."""
 import numpy as np
 import pytest
@@ -63,16 +64,20 @@ def test_uneven_batch_all_configurations(gpu, orc, uneven, spinor):
         for mapping in (1, 3):
             for persistent in (0, 1):
                 for wpb in (1, 2, 4):
-                    print(f"config prec={prec} spinor={spinor} mapping={mapping} persistent={persistent} wpb={wpb}",
-                          flush=True)
-                    plan.set_config(mapping, persistent, wpb)
-                    plan.run()
-                    gs, _ = plan.read()
-                    assert relerr(gs, want) < tol
-                    if ref is None:
-                        ref = gs
-                    else:   # the configuration only reorders whole tasks: same sums
-                        assert relerr(gs, ref) < 1e-14
+                    for chunks in (1, 3, 4):
+                        print(f"config prec={prec} spinor={spinor} mapping={mapping} persistent={persistent} "
+                              f"wpb={wpb} chunks={chunks}", flush=True)
+                        plan.set_config(mapping, persistent, wpb, chunks)
+                        assert plan.config() == (mapping, persistent, wpb, chunks)
+                        plan.run()
+                        gs, _ = plan.read()
+                        assert relerr(gs, want) < tol
+                        if ref is None:
+                            ref = gs
+                        elif chunks == 1 or mapping == 3:   # whole tasks reordered: same sums
+                            assert relerr(gs, ref) < 1e-14
+                        else:   # chunk sums added in order: rounding of the FP64 sums only
+                            assert relerr(gs, ref) < 1e-12
         assert plan.nonfinite() == 0
         plan.close()
 
@@ -83,8 +88,8 @@ def test_autotune_and_graph_replay(gpu, orc, uneven):
     plan = gpu.Plan(b, L)
     print("autotune", flush=True)
     choice = plan.autotune()
-    m, pe, w = plan.config()
-    assert choice == 100 * m + 10 * pe + w and m in (1, 2, 3)
+    m, pe, w, c = plan.config()
+    assert choice == 1000 * m + 100 * pe + 10 * w + c and m in (1, 2, 3)
     tuned, _ = plan.read()
     plan.run()
     eager, _ = plan.read()

@@ -39,8 +39,16 @@ def run_case(wfl, orc, loops, geoms, ogeoms, mode, nsub, spinor, prec, mapping, 
     b = wfl.Batch(list(geoms), tg, tcx, tcy, tT, mode=mode, precision=prec, nsub=nsub, spinor=spinor,
                   group=group, splines=list(splines), want_perloop=True, mapping=mapping)
     print(f"case mode={mode} nsub={nsub} spinor={spinor} prec={prec} mapping={mapping} ntup={len(tg)}", flush=True)
-    gs, pl = wfl.eval_batch(b, L)
+    plan = wfl.Plan(b, L)
+    try:
+        plan.run()
+        gs, pl = plan.read()
+        nonfinite = plan.nonfinite()
+    finally:
+        plan.close()
     rgs, rpl = orc.eval_batch(ogeoms, mode, nsub, spinor, x, y, group, tg, tcx, tcy, tT)
+    assert np.all(np.isfinite(gs)) and np.all(np.isfinite(pl))
+    assert nonfinite == 0      # the kernels' own count of non-finite W - 1
     return gs, pl, rgs, rpl
 
 
@@ -103,25 +111,61 @@ def test_lattice_images(gpu, orc, loops, prec, tol):
         assert e1 < tol and e2 < tol
 
 
-def gaussian_spline(wfl, n=512, rho_max=20.0, l2=1.0):
-    rho = (np.arange(n) + 1) * (rho_max / n)        # periodic.c:129-151 knot layout
-    knots = (rho * rho).astype(np.float32)
+def gaussian_spline(wfl, n=512, rho_max=20.0, l2=1.0, rng=None):
+    """Knots rho^2 of a Gaussian tube: the periodic.c:129-151 layout (rho
+    uniform: every interval within two of its bucket start), or, with rng,
+    knots at random positions with a tight cluster (the lookup's scan)."""
+    if rng is None:
+        rho = (np.arange(n) + 1) * (rho_max / n)
+        knots = (rho * rho).astype(np.float32)
+    else:   # random positions and a cluster of 8 knots 2e-4 apart at rho^2 = 0.7 rho_max^2
+        rho = rng.uniform(0.0, rho_max, n)
+        c = 0.7 * rho_max * rho_max
+        knots = np.unique(np.concatenate([(rho * rho), c + 2e-4 * np.arange(8)]).astype(np.float32))
     fli = -np.expm1(-knots.astype(np.float64) / l2)
     return knots, wfl.spline_build(fli, knots)
 
 
 @pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 5e-5)])
 @pytest.mark.parametrize("mode", [0, 1])
-def test_spline_profile(gpu, orc, loops, prec, tol, mode):
+@pytest.mark.parametrize("mapping", [1, 2])
+def test_spline_profile(gpu, orc, loops, prec, tol, mode, mapping):
+    """Spline-only batch: the tables are staged in LDS."""
     knots, coefs = gaussian_spline(gpu)
     geoms = [gpu.Geometry(kind=gpu.SPLINE, F=1.0, l2=1.0, a=1.0, spline_table=0)]
     og = [orc.Geom(orc.SPLINE, F=1.0, l2=1.0, tubedist=1.0, knots=knots, coefs=coefs)]
     for spinor in (0, 1):
-        gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, mode, 2, spinor, prec, 2,
+        gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, mode, 2, spinor, prec, mapping,
                                     tuples([0.0, 1.2], [0.02, 0.5, 3.0]), splines=[(knots, coefs)])
         e1, e2 = relerr(pl, rpl), relerr(gs, rgs)
         print(f"  spinor {spinor}: perloop {e1:.2e} gsum {e2:.2e}")
         assert e1 < tol and e2 < tol
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 5e-5)])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_spline_irregular_knots_and_mixed_profiles(gpu, orc, loops, prec, tol, mode):
+    """Two spline tables (one with knots at random positions, so the bucket
+    scans move) beside a Gaussian geometry: the run-time-profile kernels,
+    tables read from global memory; nsub = 4."""
+    rng = np.random.default_rng(7)
+    k1, c1 = gaussian_spline(gpu, n=300, rho_max=12.0, l2=0.5, rng=rng)
+    k2, c2 = gaussian_spline(gpu, n=64, rho_max=8.0, l2=2.0)
+    geoms = [gpu.Geometry(kind=gpu.SPLINE, F=1.0, l2=0.5, a=1.0, spline_table=0),
+             gpu.Geometry(kind=gpu.GAUSSIAN, F=0.5, l2=1.0),
+             gpu.Geometry(kind=gpu.SPLINE, F=2.0, l2=2.0, a=1.0, spline_table=1)]
+    og = [orc.Geom(orc.SPLINE, F=1.0, l2=0.5, tubedist=1.0, knots=k1, coefs=c1),
+          orc.Geom(orc.GAUSSIAN, F=0.5, l2=1.0),
+          orc.Geom(orc.SPLINE, F=2.0, l2=2.0, tubedist=1.0, knots=k2, coefs=c2)]
+    parts = [tuples([0.0, 0.8], [0.03, 0.7, 5.0], geom=g) for g in range(3)]
+    tup = tuple(np.concatenate([p[k] for p in parts]) for k in range(4))
+    for spinor in (0, 1):
+        for mapping in (1, 2):
+            gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, mode, 4, spinor, prec, mapping, tup,
+                                        splines=[(k1, c1), (k2, c2)])
+            e1, e2 = relerr(pl, rpl), relerr(gs, rgs)
+            print(f"  spinor {spinor} mapping {mapping}: perloop {e1:.2e} gsum {e2:.2e}")
+            assert e1 < tol and e2 < tol
 
 
 @pytest.mark.parametrize("kind", ["STEP", "SMOOTH", "QUADRATIC", "PERIODIC", "FIXFLUX", "FIXFLUX_FF"])
@@ -135,3 +179,19 @@ def test_other_profiles_fp64(gpu, orc, loops, kind, mode):
     e1, e2 = relerr(pl, rpl), relerr(gs, rgs)
     print(f"  {kind}: perloop {e1:.2e} gsum {e2:.2e}")
     assert e1 < 1e-9 and e2 < 1e-9
+
+
+@pytest.mark.parametrize("kind", ["STEP", "SMOOTH", "QUADRATIC", "PERIODIC", "FIXFLUX", "FIXFLUX_FF"])
+def test_other_profiles_mixed(gpu, orc, loops, kind):
+    """FP32 node arithmetic of the run-time-profile kernels (physical
+    segments).  Bar: 1e-4 of the case's largest per-loop value (the FP32
+    special-function forms of these profiles carry ~1e-6 relative error per
+    node; the sums are FP64)."""
+    k = getattr(gpu, kind)
+    geoms = [gpu.Geometry(kind=k, F=0.5, l2=0.64, a=2.0, lam_min=0.2)]
+    og = [orc.Geom(getattr(orc, kind), F=0.5, l2=0.64, tubedist=2.0, lmin=0.2)]
+    gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, 1, 2, 1, gpu.MIXED, 1,
+                                tuples([0.0, 0.9], [0.03, 0.6]))
+    e1, e2 = relerr(pl, rpl), relerr(gs, rgs)
+    print(f"  {kind}: perloop {e1:.2e} gsum {e2:.2e}")
+    assert e1 < 1e-4 and e2 < 1e-4

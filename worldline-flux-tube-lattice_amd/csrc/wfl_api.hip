@@ -153,6 +153,18 @@ __global__ void k_finalize_groups(const double* __restrict__ perloop, const DevT
   }
 }
 
+// Group sums from chunk sums (lane-per-tuple mapping with each loop group
+// split over cpg tasks): one thread per (tuple, group), chunks in order.
+__global__ void k_sum_slots(const double* __restrict__ part, long long n, int cpg, double* __restrict__ gsum) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const double* p = part + idx * cpg * 3;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int c = 0; c < cpg; ++c) { s0 += p[3 * c]; s1 += p[3 * c + 1]; s2 += p[3 * c + 2]; }
+  double* o = gsum + idx * 3;
+  o[0] = s0; o[1] = s1; o[2] = s2;
+}
+
 // ------------------------------------------------------ loop ensembles
 struct wfl_loops_s {
   int Nl = 0, N = 0;
@@ -278,13 +290,135 @@ static double unit_scale(const wfl_geometry& g, bool scaled) {
   return (double)(float)sqrt(WFL_LOG2E / g.l2);
 }
 
-static void fill_geom(const wfl_geometry& g, int spline_off, int spline_n, bool scaled, DevGeom& d) {
+// ------------------------------------------------------ spline blob
+// All spline tables of a launch in one float4 buffer:
+//   [coefficient rows x (n + 1) per table][bucket records x btot][knots (float), padded]
+// Rows: 0 = at or below the first knot, j + 1 = interval j, n = at or beyond
+// the last knot; the outer rows depend on the segment mode (zero in
+// reference mode; in physical mode the line s u with s = fl32(c0/k0), or c1
+// if k0 = 0, and the constant c0 of the last knot).
+// Table t has nb equal buckets over r = sqrt(rho^2) in [sqrt(k0), sqrt(k_{n-1})].
+// Bucket b holds (s, k[s], k[s+1], k[s+2]) (entries past n - 2 are +inf),
+// s = the interval of the lowest u that the kernel's FP32 bucket arithmetic
+// can place in b.  The table is "fast" when the interval of the highest such
+// u is at most s + 2 for every bucket; nb is the smallest of n/2, n, 2n, 4n,
+// 8n (at most 16384) for which it is, else 2n with the scan.
+struct SplineTab {
+  int cf, rec, kn, n, nb, fast;
+  float r0, inv_w, k0, kl;
+};
+struct SplineBlob {
+  std::vector<float4> data;
+  std::vector<SplineTab> tab;
+};
+
+// largest j <= n - 2 with k[j] <= v, 0 if none
+static int spline_interval_host(const float* k, int n, double v) {
+  int lo = 0, hi = n - 1;
+  if (!((double)k[0] <= v)) return 0;
+  if ((double)k[n - 1] <= v) return n - 2;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if ((double)k[mid] <= v) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+static bool spline_buckets(const float* k, int n, int nb, float r0, float inv_w, double margin, std::vector<int>& start) {
+  start.assign(nb, 0);
+  bool fast = true;
+  for (int b = 0; b < nb; ++b) {
+    double e0 = (double)r0 + (b - margin) / (double)inv_w, e1 = (double)r0 + (b + 1 + margin) / (double)inv_w;
+    e0 = std::max(e0, 0.0);
+    const int s = spline_interval_host(k, n, e0 * e0), h = spline_interval_host(k, n, e1 * e1);
+    start[b] = s;
+    if (h - s > 2) fast = false;
+  }
+  return fast;
+}
+
+static void build_spline_blob(int ntab, const float* const* rho2, const float* const* coefs, const int* n, int mode,
+                              SplineBlob& sb) {
+  sb = SplineBlob();
+  int rows = 0, btot = 0, ktot = 0;
+  std::vector<std::vector<int>> starts(ntab);
+  for (int t = 0; t < ntab; ++t) {
+    const int nt = n[t];
+    const float* k = rho2[t];
+    SplineTab st{};
+    st.n = nt;
+    st.k0 = k[0]; st.kl = k[nt - 1];
+    st.r0 = sqrtf(std::max(k[0], 0.0f));
+    const float r1 = sqrtf(std::max(k[nt - 1], 0.0f));
+    const double span = std::max((double)r1 - (double)st.r0, 1e-30);
+    // bucket position error of the kernel (FP32 sqrt, subtract, multiply), in bucket widths
+    auto margin_of = [&](int nb) { return std::max(0.01, 4e-7 * nb * (1.0 + (double)r1 / span)); };
+    st.fast = 0;
+    for (int nb : {std::max(16, nt / 2), nt, 2 * nt, 4 * nt, 8 * nt}) {
+      nb = std::max(16, std::min(16384, nb));
+      const float inv_w = (float)(nb / span);
+      if (spline_buckets(k, nt, nb, st.r0, inv_w, margin_of(nb), starts[t])) {
+        st.nb = nb; st.inv_w = inv_w; st.fast = 1;
+        break;
+      }
+      if (nb == 16384) break;
+    }
+    if (!st.fast) {
+      st.nb = std::max(16, std::min(16384, 2 * nt));
+      st.inv_w = (float)(st.nb / span);
+      spline_buckets(k, nt, st.nb, st.r0, st.inv_w, margin_of(st.nb), starts[t]);
+    }
+    st.cf = rows;
+    rows += nt + 1;
+    st.rec = btot;
+    btot += st.nb;
+    st.kn = ktot;
+    ktot += nt;
+    sb.tab.push_back(st);
+  }
+  const size_t kn_f4 = ((size_t)ktot + 3) / 4;
+  sb.data.assign((size_t)rows + btot + kn_f4 + 1, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+  float4* d = sb.data.data();
+  float* kn = (float*)(d + rows + btot);
+  const float inf = __builtin_inff();
+  for (int t = 0; t < ntab; ++t) {
+    SplineTab& st = sb.tab[t];
+    const int nt = st.n;
+    const float* k = rho2[t];
+    const float* c = coefs[t];
+    for (int j = 0; j + 1 < nt; ++j) d[st.cf + j + 1] = make_float4(c[4 * j], c[4 * j + 1], c[4 * j + 2], c[4 * j + 3]);
+    if (mode == WFL_MODE_PHYSICAL) {
+      const float slope = k[0] > 0.0f ? (float)((double)c[0] / (double)k[0]) : c[1];
+      d[st.cf] = make_float4(0.0f, slope, 0.0f, 0.0f);
+      d[st.cf + nt] = make_float4(c[4 * (nt - 1)], 0.0f, 0.0f, 0.0f);
+    }
+    for (int i = 0; i < nt; ++i) kn[st.kn + i] = k[i];
+    for (int b = 0; b < st.nb; ++b) {
+      const int s0 = starts[t][b];
+      float x;
+      memcpy(&x, &s0, sizeof x);
+      d[rows + st.rec + b] = make_float4(x, k[s0], s0 + 1 <= nt - 2 ? k[s0 + 1] : inf, s0 + 2 <= nt - 2 ? k[s0 + 2] : inf);
+    }
+    st.rec += rows;                        // float4 offset in the blob
+    st.kn += 4 * (rows + btot);            // float offset in the blob
+  }
+}
+
+// Largest spline blob staged in LDS (spline-only launches whose tables are all fast).
+static const size_t kSplineLdsBytes = 64 * 1024;
+
+static void fill_geom(const wfl_geometry& g, const SplineBlob* sb, bool scaled, DevGeom& d) {
   memset(&d, 0, sizeof d);
   const double sg = unit_scale(g, scaled);
   d.kind = g.kind;
   d.images = g.images;
-  d.spline_off = spline_off;
-  d.spline_n = spline_n;
+  if (g.kind == WFL_SPLINE && sb) {
+    const SplineTab& t = sb->tab[sb->tab.size() == 1 ? 0 : g.spline_table];
+    d.spline_cf = t.cf; d.spline_rec = t.rec; d.spline_kn = t.kn;
+    d.spline_n = t.n; d.spline_nb = t.nb; d.spline_fast = t.fast;
+    d.spline_r0 = t.r0; d.spline_inv_w = t.inv_w;
+    d.spline_k0 = t.k0; d.spline_kl = t.kl;
+  }
   d.Ff = (float)g.F;       d.F = d.Ff;
   d.l2 = (sg != 1.0) ? WFL_LOG2E / (sg * sg) : (double)(float)g.l2;
   d.l2f = (float)d.l2;
@@ -322,7 +456,7 @@ extern "C" int wfl_effective_tuple(const wfl_geometry* g, double cx, double cy, 
   if (!g || !out || !(g->l2 > 0.0) || !(T > 0.0)) { wfl_set_error("wfl_effective_tuple: bad arguments"); return WFL_ERR_ARG; }
   DevGeom dg;
   DevTuple dt;
-  fill_geom(*g, 0, 0, true, dg);
+  fill_geom(*g, nullptr, true, dg);
   fill_tuple(unit_scale(*g, true), dg, 0, cx, cy, sqrt(T), 0, dt);
   out[0] = dg.l2; out[1] = dg.F; out[2] = dt.cxd; out[3] = dt.cyd; out[4] = dt.sd; out[5] = dg.lattice;
   return WFL_OK;
@@ -350,13 +484,19 @@ struct wfl_plan_s {
   wfl_batch b{};
   wfl_loops* L = nullptr;
   int ngroups = 0, mapping = 1, prof_gauss = 1, ntasks = 0;
+  int cpg = 1;                   // tasks per loop group (lane-per-tuple mappings)
+  int ntasks1 = 0;               // tasks with one task per loop group
+  std::vector<int> lg_nloops, lg_images;   // per lane group: loops, image tubes
+  size_t tasks_cap = 0;          // capacity of d_tasks
+  double* d_gpart = nullptr;     // chunk sums [ntup][ngroups][cpg][3] when cpg > 1
+  size_t gpart_cap = 0;
   std::vector<DevTuple> tups;
   DevTuple* d_tups = nullptr;
   DevGeom* d_geoms = nullptr;
   int* d_lanes = nullptr;
   Task* d_tasks = nullptr;
-  float* d_knots = nullptr;
-  float4* d_coefs = nullptr;
+  float4* d_sblob = nullptr;     // spline blob
+  int sblob_f4 = 0, lds_splines = 0;
   double* d_gsum = nullptr;
   double* d_perloop = nullptr;
   double* d_wm1 = nullptr;
@@ -378,7 +518,7 @@ extern "C" void wfl_plan_destroy(wfl_plan* p) {
   if (p->gstream) (void)hipStreamDestroy(p->gstream);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
-  void* bufs[] = {p->d_tups, p->d_geoms, p->d_lanes, p->d_tasks, p->d_knots, p->d_coefs,
+  void* bufs[] = {p->d_tups, p->d_geoms, p->d_lanes, p->d_tasks, p->d_sblob, p->d_gpart,
                   p->d_gsum, p->d_perloop, p->d_wm1, p->d_nonfinite, p->d_queue};
   for (void* b : bufs) if (b) (void)hipFree(b);
   delete p;
@@ -389,6 +529,61 @@ static int upload(T** dst, const T* src, size_t n) {
   if (n == 0) n = 1;
   HIP_TRY(hipMalloc((void**)dst, sizeof(T) * n));
   if (src) HIP_TRY(hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice));
+  return WFL_OK;
+}
+
+// Compute units of the current device.
+static int device_cus() {
+  int dev = 0, ncu = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) ncu = 256;
+  return ncu;
+}
+
+// Tasks of the lane-per-tuple mapping with each loop group split into cpg
+// chunks of ceil(group / cpg) loops (cpg = 1: one task per lane group and
+// loop group), longest first (loops x image tubes) so the tail of the launch
+// is short.
+static std::vector<Task> plan_task_list(const wfl_plan* p, int cpg) {
+  const int group = p->b.group, chunk = (group + cpg - 1) / cpg;
+  const int nlg = (int)p->lg_nloops.size();
+  std::vector<Task> tasks;
+  for (int q = 0; q < p->ngroups; ++q)
+    for (int c = 0; c < cpg; ++c)
+      for (int lg = 0; lg < nlg; ++lg) {
+        int l0 = q * group + c * chunk;
+        int l1 = std::min(std::min(l0 + chunk, (q + 1) * group), p->lg_nloops[lg]);
+        if (l1 > l0) tasks.push_back(Task{lg, l0, l1 - l0, q * cpg + c});
+      }
+  std::stable_sort(tasks.begin(), tasks.end(), [&](const Task& u, const Task& v) {
+    return (long long)u.nloop * p->lg_images[u.lanegroup] > (long long)v.nloop * p->lg_images[v.lanegroup];
+  });
+  return tasks;
+}
+
+// Installs the task list for cpg chunks per group (and the chunk-sum buffer,
+// cleared: slots that no task writes must read as zero).
+static int plan_set_tasks(wfl_plan* p, int cpg) {
+  std::vector<Task> tasks = plan_task_list(p, cpg);
+  if (tasks.size() > p->tasks_cap || !p->d_tasks) {
+    if (p->d_tasks) (void)hipFree(p->d_tasks);
+    p->d_tasks = nullptr; p->tasks_cap = 0;
+    HIP_TRY(hipMalloc(&p->d_tasks, sizeof(Task) * std::max<size_t>(tasks.size(), 1)));
+    p->tasks_cap = std::max<size_t>(tasks.size(), 1);
+  }
+  if (!tasks.empty()) HIP_TRY(hipMemcpy(p->d_tasks, tasks.data(), sizeof(Task) * tasks.size(), hipMemcpyHostToDevice));
+  p->ntasks = (int)tasks.size();
+  if (cpg > 1) {
+    const size_t n = (size_t)std::max(p->b.ntup, 1) * p->ngroups * cpg * 3;
+    if (n > p->gpart_cap) {
+      if (p->d_gpart) (void)hipFree(p->d_gpart);
+      p->d_gpart = nullptr; p->gpart_cap = 0;
+      HIP_TRY(hipMalloc(&p->d_gpart, sizeof(double) * n));
+      p->gpart_cap = n;
+    }
+    HIP_TRY(hipMemset(p->d_gpart, 0, sizeof(double) * n));
+  }
+  p->cpg = cpg;
   return WFL_OK;
 }
 
@@ -424,27 +619,29 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
   const int Nl = L->Nl, N = L->N;
   p->ngroups = (Nl + b->group - 1) / b->group;
 
-  // splines: concatenated knots / coefficients
-  std::vector<int> sp_off(b->nspline > 0 ? b->nspline : 1, 0);
-  std::vector<float> knots;
-  std::vector<float4> coefs;
-  for (int s = 0; s < b->nspline; ++s) {
-    sp_off[s] = (int)knots.size();
-    const wfl_spline_table& t = b->splines[s];
-    for (int i = 0; i < t.n; ++i) {
-      knots.push_back(t.rho2[i]);
-      coefs.push_back(make_float4(t.coefs[4 * i], t.coefs[4 * i + 1], t.coefs[4 * i + 2], t.coefs[4 * i + 3]));
+  // splines: one blob of every table
+  SplineBlob sblob;
+  {
+    std::vector<const float*> kp, cp;
+    std::vector<int> np;
+    for (int s = 0; s < b->nspline; ++s) {
+      kp.push_back(b->splines[s].rho2); cp.push_back(b->splines[s].coefs); np.push_back(b->splines[s].n);
     }
+    build_spline_blob(b->nspline, kp.data(), cp.data(), np.data(), b->mode, sblob);
   }
   std::vector<DevGeom> geoms(b->ngeom);
   p->prof_gauss = 1;
+  bool all_spline = true;
   for (int g = 0; g < b->ngeom; ++g) {
     const wfl_geometry& G = b->geoms[g];
-    int off = 0, n = 0;
-    if (G.kind == WFL_SPLINE) { off = sp_off[G.spline_table]; n = b->splines[G.spline_table].n; }
-    fill_geom(G, off, n, true, geoms[g]);
+    fill_geom(G, &sblob, true, geoms[g]);
     if (G.kind != WFL_GAUSSIAN) p->prof_gauss = 0;
+    if (G.kind != WFL_SPLINE) all_spline = false;
   }
+  p->sblob_f4 = (int)sblob.data.size();
+  bool all_fast = true;
+  for (const SplineTab& t : sblob.tab) all_fast = all_fast && t.fast;
+  p->lds_splines = all_spline && all_fast && sblob.data.size() * sizeof(float4) <= kSplineLdsBytes;
   // tuples: FP32 kernel-unit values and the effective doubles
   p->tups.resize(b->ntup);
   for (int t = 0; t < b->ntup; ++t) {
@@ -476,23 +673,15 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
     k = e;
   }
   int nlg = (int)lg_nloops.size();
-  std::vector<Task> tasks;
-  for (int q = 0; q < p->ngroups; ++q)
-    for (int lg = 0; lg < nlg; ++lg) {
-      int l0 = q * b->group, l1 = std::min(l0 + b->group, lg_nloops[lg]);
-      if (l1 > l0) tasks.push_back(Task{lg, l0, l1 - l0, q});
-    }
-  // longest tasks first (loops x image tubes), so the tail of the launch is short
   std::vector<int> lg_images(nlg, 1);
   for (int lg = 0; lg < nlg; ++lg) {
     int t0 = lanes[(size_t)lg * 64];
     const wfl_geometry& G = b->geoms[p->tups[t0].geom];
     if (G.lattice > 0.0) lg_images[lg] = (2 * G.images + 1) * (2 * G.images + 1);
   }
-  std::stable_sort(tasks.begin(), tasks.end(), [&](const Task& u, const Task& v) {
-    return (long long)u.nloop * lg_images[u.lanegroup] > (long long)v.nloop * lg_images[v.lanegroup];
-  });
-  p->ntasks = (int)tasks.size();
+  p->lg_nloops = lg_nloops;
+  p->lg_images = lg_images;
+  p->ntasks = p->ntasks1 = (int)plan_task_list(p, 1).size();
   // mapping (autotune heuristic: lane-per-tuple needs enough lane groups)
   int mapping = b->mapping;
   const double perloop_bytes = 48.0 * (double)std::max(b->ntup, 1) * (double)Nl;
@@ -508,9 +697,17 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
   TRY(upload(&p->d_tups, p->tups.data(), p->tups.size()));
   TRY(upload(&p->d_geoms, geoms.data(), geoms.size()));
   TRY(upload(&p->d_lanes, lanes.data(), lanes.size()));
-  TRY(upload(&p->d_tasks, tasks.data(), tasks.size()));
-  TRY(upload(&p->d_knots, knots.data(), knots.size()));
-  TRY(upload(&p->d_coefs, coefs.data(), coefs.size()));
+  {
+    // small batches: split loop groups so that the device holds about 32
+    // wavefronts per CU (the autotuner measures the alternatives)
+    int cpg = 1;
+    const long long want = 32LL * device_cus();
+    while (mapping != 2 && cpg < 32 && 2 * cpg <= b->group && (long long)p->ntasks1 * cpg < want &&
+           8.0 * 3.0 * 2 * cpg * (double)std::max(b->ntup, 1) * (double)p->ngroups <= 2.0e9)
+      cpg *= 2;
+    TRY(plan_set_tasks(p, cpg));
+  }
+  TRY(upload(&p->d_sblob, sblob.data.data(), sblob.data.size()));
   size_t ng3 = (size_t)std::max(b->ntup, 1) * p->ngroups * 3;
   TRY(upload<double>(&p->d_gsum, nullptr, ng3));
   {
@@ -546,11 +743,13 @@ static void fill_launch(const wfl_plan* p, WflLaunch& a, hipStream_t s) {
   a.rec = p->b.precision == WFL_MIXED ? (const void*)p->L->rec32 : (const void*)p->L->rec64;
   a.N = p->L->N; a.Nl = p->L->Nl;
   a.tups = p->d_tups; a.geoms = p->d_geoms; a.ntup = p->b.ntup;
-  a.lanes = p->d_lanes; a.tasks = p->d_tasks; a.ntasks = p->ntasks; a.ngroups = p->ngroups;
-  a.knots = p->d_knots; a.coefs = p->d_coefs;
+  a.lanes = p->d_lanes; a.tasks = p->d_tasks; a.ntasks = p->ntasks;
+  a.nslots = p->mapping == 1 ? p->ngroups * p->cpg : p->ngroups;
+  a.sblob = p->d_sblob; a.sblob_f4 = p->sblob_f4; a.lds_splines = p->lds_splines;
   a.prof_gauss = p->prof_gauss; a.spinor = p->b.spinor ? 1 : 0; a.mode = p->b.mode; a.nsub = p->b.nsub;
   a.split = p->mapping == 3;
-  a.gsum = p->d_gsum; a.perloop = p->d_perloop; a.wm1 = p->d_wm1;
+  a.gsum = (p->mapping == 1 && p->cpg > 1) ? p->d_gpart : p->d_gsum;
+  a.perloop = p->d_perloop; a.wm1 = p->d_wm1;
   a.nonfinite = p->d_nonfinite;
   a.wpb = p->wpb;
   a.persistent = p->persistent;
@@ -572,6 +771,12 @@ extern "C" int wfl_plan_run(wfl_plan* p, void* stream) {
   if (p->mapping == 1 || p->mapping == 3) {
     e = p->b.precision == WFL_MIXED ? wfl_launch_tuple_f32(&a) : wfl_launch_tuple_f64(&a);
     if (p->timed && e == 0) e = (int)hipEventRecord(p->ev1, s);
+    if (e == 0 && p->mapping == 1 && p->cpg > 1) {
+      long long n = (long long)p->b.ntup * p->ngroups;
+      hipLaunchKernelGGL(k_sum_slots, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, p->d_gpart, n, p->cpg,
+                         p->d_gsum);
+      e = (int)hipGetLastError();
+    }
     if (e == 0 && p->mapping == 3) {
       long long n = (long long)p->b.ntup * p->ngroups;
       hipLaunchKernelGGL(k_finalize_groups, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, s, p->d_perloop,
@@ -610,33 +815,39 @@ extern "C" int wfl_plan_run_timed(wfl_plan* p, void* stream, float* kernel_ms) {
 // ------------------------------------------------------ autotuner
 // Measured choice of the launch configuration: mapping (lane per tuple
 // inline / split finalize, wave per loop for small batches), static or
-// persistent grid, and wavefronts per block.  Every candidate run is a full,
-// valid evaluation; the fastest configuration is kept on the plan and
-// remembered for later plans of the same shape in this process.
+// persistent grid, wavefronts per block, and for the lane-per-tuple mappings
+// the number of tasks each loop group is split into (small batches: more,
+// shorter tasks fill the machine).  Every candidate run is a full, valid
+// evaluation; the fastest configuration is kept on the plan and remembered
+// for later plans of the same shape in this process.
 struct TuneKey {
-  int ntup, Nl, N, precision, spinor, mode, nsub, prof_gauss, ntasks;
+  int ntup, Nl, N, precision, spinor, mode, nsub, prof_gauss, lds_splines, ntasks1;
   bool operator==(const TuneKey& o) const { return memcmp(this, &o, sizeof o) == 0; }
 };
-struct TuneVal { int mapping, persistent, wpb; };
+struct TuneVal { int mapping, persistent, wpb, cpg; };
 static std::vector<std::pair<TuneKey, TuneVal>> g_tuned;
 
-static int plan_apply(wfl_plan* p, int mapping, int persistent, int wpb) {
+static int plan_apply(wfl_plan* p, int mapping, int persistent, int wpb, int cpg) {
   if (mapping == 3 && !p->d_perloop) {
     size_t n6 = (size_t)std::max(p->b.ntup, 1) * p->L->Nl * 6;
     HIP_TRY(hipMalloc(&p->d_perloop, sizeof(double) * n6));
   }
   if (mapping == 2 && !p->d_wm1) HIP_TRY(hipMalloc(&p->d_wm1, sizeof(double) * (size_t)std::max(p->b.ntup, 1) * p->L->Nl * 3));
-  int dev = 0, ncu = 0;
-  (void)hipGetDevice(&dev);
-  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) ncu = 256;
+  if (mapping == 2) cpg = 1;
+  if (cpg != p->cpg) {
+    int rc = plan_set_tasks(p, cpg);
+    if (rc) return rc;
+  }
   p->mapping = mapping;
   p->persistent = persistent;
   p->wpb = wpb;
-  p->grid = ncu * 32 / wpb;
+  p->grid = device_cus() * 32 / wpb;
   if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
   if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
   return WFL_OK;
 }
+
+static int tune_code(const TuneVal& v) { return v.mapping * 1000 + v.persistent * 100 + v.wpb * 10 + v.cpg; }
 
 extern "C" int wfl_plan_autotune(wfl_plan* p, void* stream, int* chosen) {
   if (!p) { wfl_set_error("wfl_plan_autotune: NULL plan"); return WFL_ERR_STATE; }
@@ -644,23 +855,32 @@ extern "C" int wfl_plan_autotune(wfl_plan* p, void* stream, int* chosen) {
   memset(&key, 0, sizeof key);
   key.ntup = p->b.ntup; key.Nl = p->L->Nl; key.N = p->L->N; key.precision = p->b.precision;
   key.spinor = p->b.spinor; key.mode = p->b.mode; key.nsub = p->b.nsub; key.prof_gauss = p->prof_gauss;
-  key.ntasks = p->ntasks;
+  key.lds_splines = p->lds_splines;
+  key.ntasks1 = p->ntasks1;
   for (auto& kv : g_tuned)
     if (kv.first == key) {
-      int rc = plan_apply(p, kv.second.mapping, kv.second.persistent, kv.second.wpb);
+      int rc = plan_apply(p, kv.second.mapping, kv.second.persistent, kv.second.wpb, kv.second.cpg);
       if (rc) return rc;
-      if (chosen) *chosen = kv.second.mapping * 100 + kv.second.persistent * 10 + kv.second.wpb;
+      if (chosen) *chosen = tune_code(kv.second);
       p->tuned = true;
       return WFL_OK;
     }
+  // chunked groups only when one task per (lane group, loop group) gives
+  // fewer than 4 wavefronts per SIMD
+  std::vector<int> cpgs{1};
+  const bool few = (long long)p->ntasks1 < 16LL * device_cus();
+  for (int c : {2, 4, 8, 16, 32})
+    if (few && c <= p->b.group &&
+        8.0 * 3.0 * c * (double)std::max(p->b.ntup, 1) * (double)p->ngroups <= 2.0e9) cpgs.push_back(c);
+  const double perloop_bytes = 48.0 * (double)std::max(p->b.ntup, 1) * (double)p->L->Nl;
   std::vector<TuneVal> cand;
-  for (int pers = 0; pers <= 1; ++pers)
-    for (int wpb : {2, 4}) {
-      cand.push_back(TuneVal{1, pers, wpb});
-      const double perloop_bytes = 48.0 * (double)std::max(p->b.ntup, 1) * (double)p->L->Nl;
-      if (p->prof_gauss && perloop_bytes <= 24.0e9) cand.push_back(TuneVal{3, pers, wpb});
-    }
-  if (p->b.ntup < 2048) cand.push_back(TuneVal{2, 0, 4});
+  for (int c : cpgs)
+    for (int pers = 0; pers <= 1; ++pers)
+      for (int wpb : {2, 4}) {
+        cand.push_back(TuneVal{1, pers, wpb, c});
+        if (p->prof_gauss && perloop_bytes <= 24.0e9) cand.push_back(TuneVal{3, pers, wpb, c});
+      }
+  if (p->b.ntup < 2048 || few) cand.push_back(TuneVal{2, 0, 4, 1});
   TuneVal best = cand[0];
   float best_ms = 1e30f;
   hipEvent_t e0, e1;
@@ -668,7 +888,7 @@ extern "C" int wfl_plan_autotune(wfl_plan* p, void* stream, int* chosen) {
   HIP_TRY(hipEventCreate(&e1));
   int rc = WFL_OK;
   for (const TuneVal& c : cand) {
-    rc = plan_apply(p, c.mapping, c.persistent, c.wpb);
+    rc = plan_apply(p, c.mapping, c.persistent, c.wpb, c.cpg);
     if (rc) break;
     float ms = 0.0f;
     (void)hipEventRecord(e0, (hipStream_t)stream);
@@ -685,30 +905,32 @@ extern "C" int wfl_plan_autotune(wfl_plan* p, void* stream, int* chosen) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
-  rc = plan_apply(p, best.mapping, best.persistent, best.wpb);
+  rc = plan_apply(p, best.mapping, best.persistent, best.wpb, best.cpg);
   if (rc) return rc;
   rc = wfl_plan_run(p, stream);      // leave the plan's outputs from the chosen configuration
   if (rc) return rc;
   g_tuned.push_back({key, best});
   p->tuned = true;
-  if (chosen) *chosen = best.mapping * 100 + best.persistent * 10 + best.wpb;
+  if (chosen) *chosen = tune_code(best);
   return WFL_OK;
 }
 
-extern "C" int wfl_plan_config(const wfl_plan* p, int* mapping, int* persistent, int* wpb) {
+extern "C" int wfl_plan_config(const wfl_plan* p, int* mapping, int* persistent, int* wpb, int* chunks) {
   if (!p) { wfl_set_error("wfl_plan_config: NULL plan"); return WFL_ERR_ARG; }
   if (mapping) *mapping = p->mapping;
   if (persistent) *persistent = p->persistent;
   if (wpb) *wpb = p->wpb;
+  if (chunks) *chunks = p->mapping == 2 ? 1 : p->cpg;
   return WFL_OK;
 }
 
-extern "C" int wfl_plan_set_config(wfl_plan* p, int mapping, int persistent, int wpb) {
-  if (!p || mapping < 1 || mapping > 3 || (persistent != 0 && persistent != 1) || (wpb != 1 && wpb != 2 && wpb != 4)) {
+extern "C" int wfl_plan_set_config(wfl_plan* p, int mapping, int persistent, int wpb, int chunks) {
+  if (!p || mapping < 1 || mapping > 3 || (persistent != 0 && persistent != 1) || (wpb != 1 && wpb != 2 && wpb != 4) ||
+      chunks < 1 || chunks > 64 || chunks > p->b.group) {
     wfl_set_error("wfl_plan_set_config: bad configuration"); return WFL_ERR_ARG;
   }
   if (mapping == 3 && !p->prof_gauss) { wfl_set_error("wfl_plan_set_config: split finalize needs the Gaussian fast path"); return WFL_ERR_ARG; }
-  return plan_apply(p, mapping, persistent, wpb);
+  return plan_apply(p, mapping, persistent, wpb, chunks);
 }
 
 extern "C" int wfl_plan_graph_run(wfl_plan* p, void* stream) {
@@ -772,6 +994,8 @@ static struct {
   DevTuple* d_tup = nullptr;
   DevGeom* d_geom = nullptr;
   unsigned long long* d_nf = nullptr;
+  float4* d_sblob = nullptr;     // spline blob built from the caller's tables
+  size_t sblob_cap = 0;          // in float4
 } g_ev;
 
 extern "C" void wfl_internal_ev_release(void) {
@@ -779,7 +1003,9 @@ extern "C" void wfl_internal_ev_release(void) {
   if (g_ev.d_tup) (void)hipFree(g_ev.d_tup);
   if (g_ev.d_geom) (void)hipFree(g_ev.d_geom);
   if (g_ev.d_nf) (void)hipFree(g_ev.d_nf);
+  if (g_ev.d_sblob) (void)hipFree(g_ev.d_sblob);
   g_ev.L = nullptr; g_ev.d_tup = nullptr; g_ev.d_geom = nullptr; g_ev.d_nf = nullptr;
+  g_ev.d_sblob = nullptr; g_ev.sblob_cap = 0;
   g_ev.aos = nullptr; g_ev.Nl = g_ev.N = 0;
 }
 
@@ -808,8 +1034,28 @@ extern "C" int wfl_internal_ev_launch(const void* dev_aos, int Nl, int N, const 
   gg.lattice = 0.0;
   gg.images = 0;
   const bool scaled = precision == WFL_MIXED;
+  // spline: the caller's device tables (read back every call: the caller may
+  // rewrite them in place) as a blob with its bucket table
+  SplineBlob sb;
+  if (g->kind == WFL_SPLINE) {
+    std::vector<float> kn((size_t)nspline), cf((size_t)nspline * 4);
+    HIP_TRY(hipMemcpy(kn.data(), knots_d, sizeof(float) * kn.size(), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cf.data(), coefs_d, sizeof(float) * cf.size(), hipMemcpyDeviceToHost));
+    for (int i = 1; i < nspline; ++i)
+      if (!(kn[i] > kn[i - 1])) { wfl_set_error("EV: spline knots not increasing at %d", i); return WFL_ERR_ARG; }
+    const float* kp = kn.data();
+    const float* cp = cf.data();
+    build_spline_blob(1, &kp, &cp, &nspline, mode, sb);
+    if (sb.data.size() > g_ev.sblob_cap) {
+      if (g_ev.d_sblob) (void)hipFree(g_ev.d_sblob);
+      g_ev.d_sblob = nullptr; g_ev.sblob_cap = 0;
+      HIP_TRY(hipMalloc(&g_ev.d_sblob, sizeof(float4) * sb.data.size()));
+      g_ev.sblob_cap = sb.data.size();
+    }
+    HIP_TRY(hipMemcpy(g_ev.d_sblob, sb.data.data(), sizeof(float4) * sb.data.size(), hipMemcpyHostToDevice));
+  }
   DevGeom dg;
-  fill_geom(gg, 0, g->kind == WFL_SPLINE ? nspline : 0, scaled, dg);
+  fill_geom(gg, g->kind == WFL_SPLINE ? &sb : nullptr, scaled, dg);
   DevTuple dt;
   fill_tuple(unit_scale(gg, scaled), dg, 0, (double)cx, (double)cy, (double)rtT, Nl, dt);
   HIP_TRY(hipMemcpy(g_ev.d_geom, &dg, sizeof dg, hipMemcpyHostToDevice));
@@ -818,7 +1064,10 @@ extern "C" int wfl_internal_ev_launch(const void* dev_aos, int Nl, int N, const 
   memset(&a, 0, sizeof a);
   a.rec = precision == WFL_MIXED ? (const void*)g_ev.L->rec32 : (const void*)g_ev.L->rec64;
   a.N = N; a.Nl = Nl; a.tups = g_ev.d_tup; a.geoms = g_ev.d_geom; a.ntup = 1;
-  a.knots = knots_d; a.coefs = (const float4*)coefs_d;
+  if (g->kind == WFL_SPLINE) {
+    a.sblob = g_ev.d_sblob; a.sblob_f4 = (int)sb.data.size();
+    a.lds_splines = sb.tab[0].fast && sb.data.size() * sizeof(float4) <= kSplineLdsBytes;
+  }
   a.prof_gauss = g->kind == WFL_GAUSSIAN; a.spinor = fermion == 1; a.mode = mode; a.nsub = nsub;
   a.ws_scal = (float4*)ws_scal_d; a.ws_ferm = fermion == 1 ? (float4*)ws_ferm_d : nullptr;
   a.nonfinite = g_ev.d_nf;

@@ -27,7 +27,10 @@
 
 namespace wfl {
 
-struct Task { int lanegroup, loop0, nloop, group; };
+// One wavefront's work: a lane group (64 tuples of one geometry) on loops
+// [loop0, loop0 + nloop), summed into output slot `slot` (the loop group, or
+// one of its chunks when groups are split into several tasks).
+struct Task { int lanegroup, loop0, nloop, slot; };
 
 // values per point record (w, dw, e and two pad slots: 32 B in FP32)
 constexpr int REC = 8;
@@ -57,15 +60,14 @@ template <class R> struct LaneC {
 // --------------------------------------------------------- node evaluation
 template <class R, int PROF>
 __device__ __forceinline__ R node_g(R u, const LaneC<R>& lc, int kind, const GP<R>& gp, int mode,
-                                    const float* __restrict__ kn, const float4* __restrict__ cf, int ns,
-                                    bool spinor, R& fp) {
+                                    const SplineRef& sp, bool spinor, R& fp) {
   if (PROF == P_GAUSSIAN) {
     R e;
     R gval = gauss_g(u, (double)lc.inv_l2, e);
     fp = e;
     return gval;
   } else {
-    return generic_g<R>(kind, gp, u, mode, spinor, kn, cf, ns, fp);
+    return generic_g<R, PROF>(kind, gp, u, mode, spinor, sp, fp);
   }
 }
 
@@ -77,8 +79,7 @@ __device__ __forceinline__ R node_g(R u, const LaneC<R>& lc, int kind, const GP<
 template <class R, int PROF, int SPINOR, int MODE, int NSUB>
 __device__ __forceinline__ void point_eval(R wx, R wy, R dx4, R dy4, R ex, R ey,
                                            R ccx, R ccy, const LaneC<R>& lc, int kind, const GP<R>& gp,
-                                           const float* __restrict__ kn, const float4* __restrict__ cf, int ns,
-                                           int nsub_rt, R& P, R& S) {
+                                           const SplineRef& sp, int nsub_rt, R& P, R& S) {
   const int n = NSUB ? NSUB : nsub_rt;
   const R tiny = sizeof(R) == 4 ? (R)1e-30 : (R)0;   // keeps 1/u finite on the axis (FP32 node)
   R zx = fma(lc.s, wx, ccx);
@@ -88,12 +89,12 @@ __device__ __forceinline__ void point_eval(R wx, R wy, R dx4, R dy4, R ex, R ey,
     R E = fma(zx, ey, -(zy * ex));        // z x e: both endpoint weights of this point
     R A = fma(zx, zx, fma(zy, zy, tiny));
     R fA, fk;
-    R gA = node_g<R, PROF>(A, lc, kind, gp, MODE, kn, cf, ns, SPINOR, fA);
+    R gA = node_g<R, PROF>(A, lc, kind, gp, MODE, sp, SPINOR, fA);
     R gin, fin;
     if (n == 2) {
       R mx = fma(lc.m8, dx4, zx), my = fma(lc.m8, dy4, zy);
       R Am = fma(mx, mx, fma(my, my, tiny));
-      gin = node_g<R, PROF>(Am, lc, kind, gp, MODE, kn, cf, ns, SPINOR, fk);
+      gin = node_g<R, PROF>(Am, lc, kind, gp, MODE, sp, SPINOR, fk);
       fin = fk + fk;
     } else {
       const R h4 = lc.s4 / (R)n;          // node k sits at z + (k h s/4)(4 dw)
@@ -102,7 +103,7 @@ __device__ __forceinline__ void point_eval(R wx, R wy, R dx4, R dy4, R ex, R ey,
         R t = (R)k * h4;
         R px = fma(t, dx4, zx), py = fma(t, dy4, zy);
         R u = fma(px, px, fma(py, py, tiny));
-        R gk = node_g<R, PROF>(u, lc, kind, gp, MODE, kn, cf, ns, SPINOR, fk);
+        R gk = node_g<R, PROF>(u, lc, kind, gp, MODE, sp, SPINOR, fk);
         gin = (k & 1) ? gin + gk : fma((R)0.5, gk, gin);      // Simpson weights 4, 2 over 4
         if (SPINOR) fin = (k & 1) ? fma((R)2, fk, fin) : fin + fk;   // and over 2
       }
@@ -121,13 +122,13 @@ __device__ __forceinline__ void point_eval(R wx, R wy, R dx4, R dy4, R ex, R ey,
     R A1 = A + (R)2 * B + C;
     A1 = A1 < lc.c8 ? lc.c8 : A1;
     R f0, f1, fk;
-    R g0 = node_g<R, PROF>(A, lc, kind, gp, MODE, kn, cf, ns, SPINOR, f0);
-    R g1 = node_g<R, PROF>(A1, lc, kind, gp, MODE, kn, cf, ns, SPINOR, f1);
+    R g0 = node_g<R, PROF>(A, lc, kind, gp, MODE, sp, SPINOR, f0);
+    R g1 = node_g<R, PROF>(A1, lc, kind, gp, MODE, sp, SPINOR, f1);
     R gs = (R)0.25 * (g0 + g1), fs = SPINOR ? (R)0.5 * (f0 + f1) : (R)0;
     if (n == 2) {
       R u = A + B + (R)0.25 * C;
       u = u < lc.c10 ? lc.c10 : u;
-      gs += node_g<R, PROF>(u, lc, kind, gp, MODE, kn, cf, ns, SPINOR, fk);
+      gs += node_g<R, PROF>(u, lc, kind, gp, MODE, sp, SPINOR, fk);
       if (SPINOR) fs = fma((R)2, fk, fs);
     } else {
       const R h = (R)1 / (R)n;
@@ -135,7 +136,7 @@ __device__ __forceinline__ void point_eval(R wx, R wy, R dx4, R dy4, R ex, R ey,
         R t = (R)k * h;
         R u = A + (R)2 * B * t + C * t * t;
         u = u < lc.c10 ? lc.c10 : u;
-        R gk = node_g<R, PROF>(u, lc, kind, gp, MODE, kn, cf, ns, SPINOR, fk);
+        R gk = node_g<R, PROF>(u, lc, kind, gp, MODE, sp, SPINOR, fk);
         gs = (k & 1) ? gs + gk : fma((R)0.5, gk, gs);
         if (SPINOR) fs = (k & 1) ? fma((R)2, fk, fs) : fs + fk;
       }
@@ -145,7 +146,9 @@ __device__ __forceinline__ void point_eval(R wx, R wy, R dx4, R dy4, R ex, R ey,
   }
 }
 
-template <class R> __device__ __forceinline__ bool finite_r(R v) { return v - v == (R)0; }
+// (a library test: v - v == 0 is not one once the compiler contracts v = a b
+// into fma(a, b, -v), the rounding error of the product)
+template <class R> __device__ __forceinline__ bool finite_r(R v) { return __builtin_isfinite(v); }
 
 // W - 1 of the three Richardson estimates from the per-loop sums
 // (Integrand.c:352-390), in FP64; the reference's cosh guard
@@ -207,6 +210,20 @@ __device__ __forceinline__ void finalize_loop(const double P[3], const double S[
   }
 }
 
+// Spline-only kernels copy the plan's spline blob to LDS once per block
+// (all threads, before any work); other kernels read it from global memory.
+template <int PROF>
+__device__ __forceinline__ const float4* stage_splines(const float4* __restrict__ sblob, int sblob_f4) {
+  if constexpr (PROF == P_SPLINE) {
+    extern __shared__ float4 wfl_lds_splines[];
+    for (int i = threadIdx.x; i < sblob_f4; i += blockDim.x) wfl_lds_splines[i] = sblob[i];
+    __syncthreads();
+    return wfl_lds_splines;
+  } else {
+    return sblob;
+  }
+}
+
 // ------------------------------------------------------ lane per tuple
 // SPLIT = 1 writes each loop's (Wsscal, Wsferm) to perloop and leaves W - 1
 // and the group sums to k_finalize_groups: the FP64 trigonometry then does
@@ -215,15 +232,15 @@ template <class R, int PROF, int SPINOR, int MODE, int NSUB, int SPLIT>
 __device__ __forceinline__ void tuple_task(
     int task_id, int lane, const R* __restrict__ rec, int N, int Nl, const DevTuple* __restrict__ tups,
     const DevGeom* __restrict__ geoms, const int* __restrict__ lanes,
-    const float* __restrict__ knots, const float4* __restrict__ coefs,
-    const Task* __restrict__ tasks, int ngroups, int nsub_rt,
+    const float4* __restrict__ sblob,
+    const Task* __restrict__ tasks, int nslots, int nsub_rt,
     double* __restrict__ gsum, double* __restrict__ perloop,
     unsigned long long* __restrict__ nonfinite) {
   const Task tk = tasks[task_id];
   const int lg = __builtin_amdgcn_readfirstlane(tk.lanegroup);
   const int loop0 = __builtin_amdgcn_readfirstlane(tk.loop0);
   const int nloop = __builtin_amdgcn_readfirstlane(tk.nloop);
-  const int grp = __builtin_amdgcn_readfirstlane(tk.group);
+  const int slot = __builtin_amdgcn_readfirstlane(tk.slot);
   const int tid_raw = lanes[lg * 64 + lane];
   const bool valid = tid_raw >= 0;
   const int tid = valid ? tid_raw : lanes[lg * 64];
@@ -235,9 +252,7 @@ __device__ __forceinline__ void tuple_task(
   const double s_phase = sizeof(R) == 4 ? (double)tp.s : tp.sd;
   const double spin_scale = (PROF == P_GAUSSIAN) ? gm.inv_l2 : 1.0;
   const int kind = gm.kind;
-  const float* kn = knots + gm.spline_off;
-  const float4* cf = coefs + gm.spline_off;
-  const int ns = gm.spline_n;
+  const SplineRef sp = spline_ref(sblob, gm);
   const int n = NSUB ? NSUB : nsub_rt;
   // image window (uniform over the lane group: one geometry per group)
   const int M = __builtin_amdgcn_readfirstlane(gm.lattice > 0.0 ? gm.images : 0);
@@ -264,8 +279,8 @@ __device__ __forceinline__ void tuple_task(
             for (int u = 0; u < 4; ++u) {
               const int c = (u & 1) ? 0 : (u == 0 ? 2 : 1);
               point_eval<R, PROF, SPINOR, MODE, NSUB>(r[REC * u + 0], r[REC * u + 1], r[REC * u + 2], r[REC * u + 3],
-                                                      r[REC * u + 4], r[REC * u + 5], ccx, ccy, lc, kind, gp, kn, cf,
-                                                      ns, nsub_rt, p[c], q[c]);
+                                                      r[REC * u + 4], r[REC * u + 5], ccx, ccy, lc, kind, gp, sp,
+                                                      nsub_rt, p[c], q[c]);
             }
           }
 #pragma unroll
@@ -288,7 +303,7 @@ __device__ __forceinline__ void tuple_task(
     }
   }
   if (!SPLIT && valid) {
-    double* o = gsum + ((size_t)tid * ngroups + grp) * 3;
+    double* o = gsum + ((size_t)tid * nslots + slot) * 3;
     o[0] = G[0]; o[1] = G[1]; o[2] = G[2];
   }
 }
@@ -309,16 +324,17 @@ template <class R, int PROF, int SPINOR, int MODE, int NSUB, int SPLIT>
 __global__ __launch_bounds__(256) void k_tuple(
     const R* __restrict__ rec, int N, int Nl, const DevTuple* __restrict__ tups,
     const DevGeom* __restrict__ geoms, const int* __restrict__ lanes,
-    const float* __restrict__ knots, const float4* __restrict__ coefs,
-    const Task* __restrict__ tasks, int ntasks, int ngroups, int nsub_rt,
+    const float4* __restrict__ sblob, int sblob_f4,
+    const Task* __restrict__ tasks, int ntasks, int nslots, int nsub_rt,
     double* __restrict__ gsum, double* __restrict__ perloop,
     unsigned long long* __restrict__ nonfinite, int wpb, int persistent, int* __restrict__ queue) {
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
+  const float4* sb = stage_splines<PROF>(sblob, sblob_f4);
   int task_id = persistent ? next_task(queue, lane) : (int)blockIdx.x * wpb + wv;
   while (task_id < ntasks) {
-    tuple_task<R, PROF, SPINOR, MODE, NSUB, SPLIT>(task_id, lane, rec, N, Nl, tups, geoms, lanes, knots, coefs,
-                                                   tasks, ngroups, nsub_rt, gsum, perloop, nonfinite);
+    tuple_task<R, PROF, SPINOR, MODE, NSUB, SPLIT>(task_id, lane, rec, N, Nl, tups, geoms, lanes, sb,
+                                                   tasks, nslots, nsub_rt, gsum, perloop, nonfinite);
     if (!persistent) break;
     task_id = next_task(queue, lane);
   }
@@ -333,10 +349,11 @@ template <class R, int PROF, int SPINOR, int MODE, int NSUB>
 __global__ __launch_bounds__(256) void k_loop(
     const R* __restrict__ rec, int N, int Nl, const DevTuple* __restrict__ tups,
     const DevGeom* __restrict__ geoms, int ntup,
-    const float* __restrict__ knots, const float4* __restrict__ coefs, int nsub_rt,
+    const float4* __restrict__ sblob, int sblob_f4, int nsub_rt,
     double* __restrict__ perloop, double* __restrict__ wm1,
     float4* __restrict__ ws_scal, float4* __restrict__ ws_ferm,
     unsigned long long* __restrict__ nonfinite, int wpb) {
+  const float4* sb = stage_splines<PROF>(sblob, sblob_f4);   // before any wavefront leaves
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long long w = (long long)blockIdx.x * wpb + wv;
   if (w >= (long long)ntup * Nl) return;
@@ -351,9 +368,7 @@ __global__ __launch_bounds__(256) void k_loop(
   const double s_phase = sizeof(R) == 4 ? (double)tp.s : tp.sd;
   const double spin_scale = (PROF == P_GAUSSIAN) ? gm.inv_l2 : 1.0;
   const int kind = gm.kind;
-  const float* kn = knots + gm.spline_off;
-  const float4* cf = coefs + gm.spline_off;
-  const int ns = gm.spline_n;
+  const SplineRef sp = spline_ref(sb, gm);
   const int n = NSUB ? NSUB : nsub_rt;
   const int M = gm.lattice > 0.0 ? gm.images : 0;
   const R lat = sizeof(R) == 4 ? (R)gm.latticef : (R)gm.lattice;
@@ -373,8 +388,8 @@ __global__ __launch_bounds__(256) void k_loop(
         const int i1 = i0 + 64 * 8 < N ? i0 + 64 * 8 : N;
         for (int i = i0; i < i1; i += 64) {
           const R* r = rl + (size_t)i * REC;
-          point_eval<R, PROF, SPINOR, MODE, NSUB>(r[0], r[1], r[2], r[3], r[4], r[5], ccx, ccy, lc, kind, gp, kn,
-                                                  cf, ns, nsub_rt, p, q);
+          point_eval<R, PROF, SPINOR, MODE, NSUB>(r[0], r[1], r[2], r[3], r[4], r[5], ccx, ccy, lc, kind, gp, sp,
+                                                  nsub_rt, p, q);
         }
         P += (double)p;
         S += (double)q;
@@ -414,8 +429,9 @@ __global__ __launch_bounds__(256) void k_loop(
 struct WflLaunch {
   const void* rec; int N, Nl;
   const wfl::DevTuple* tups; const wfl::DevGeom* geoms; int ntup;
-  const int* lanes; const wfl::Task* tasks; int ntasks; int ngroups;
-  const float* knots; const float4* coefs;
+  const int* lanes; const wfl::Task* tasks; int ntasks; int nslots;
+  const float4* sblob; int sblob_f4;          // spline blob and its size in float4
+  int lds_splines;                            // 1: spline-only kernels with the blob staged in LDS
   int prof_gauss, spinor, mode, nsub, split;
   int wpb, persistent, grid;   // wavefronts per block; persistent grid size in blocks
   int* queue;

@@ -28,7 +28,11 @@ enum { P_ANY = 99 };
 // x_cm = c'/sigma, sqrt(T) = s'/sigma, lattice = L'/sigma), which is what the
 // FP64 kernels and the oracle evaluate.
 struct DevGeom {
-  int kind, images, spline_off, spline_n;
+  int kind, images;
+  int spline_cf, spline_rec, spline_kn;             // spline blob offsets: coefs, buckets (float4), knots (float)
+  int spline_n, spline_nb, spline_fast;             // knots, buckets, 1: every interval within 2 of its bucket start
+  float spline_r0, spline_inv_w;                    // first bucket edge and 1/width in sqrt(rho^2)
+  float spline_k0, spline_kl;                       // first and last knot
   double F, l2, inv_l2, a, lmin, q1, q2, lattice;   // effective values (FP64 kernels)
   float Ff, l2f, inv_l2f, af, lminf, q1f, q2f;      // FP32 generic-profile values
   float latticef;                                   // lattice spacing in FP32 kernel units
@@ -202,51 +206,96 @@ template <class R> __device__ R fpfixflux(const GP<R>& g, R rho2) {   // intEV.c
          Math<R>::div((R)6, g.q1 * lam * g.a) * aml * bump<R>(Math<R>::div((R)2 * (rho - n * g.a), lam));
 }
 
-// Spline: interval by bisection over the sorted knots (intEV.cu:81-89
-// semantics: the largest j with knot[j] <= u), cubic by Horner.
-template <class R>
-__device__ int spline_interval(R u, const float* __restrict__ kn, int n) {
-  int lo = 0, hi = n - 1;
-  while (hi - lo > 1) {
-    int mid = (hi + lo) >> 1;
-    if (u >= (R)kn[mid]) lo = mid; else hi = mid;
-  }
-  return lo;
+// Spline tables.  A plan stores all of its tables in one blob of float4:
+//   [coefs x (n + 1) per table][bucket records x btot][knots (float) x ctot]
+// read from global memory or from a copy in LDS.  Coefficient row 0 is the
+// part at or below the first knot and row n the part at or beyond the last
+// (for the segment mode of the launch: zero in reference mode, intEV.cu:96-99;
+// in physical mode the line s u, s = fl32(c0/k0), from u = 0, and the
+// constant f(k_{n-1}) from k_{n-1}); row j + 1 is interval j.  The interval
+// of k0 < u < k_{n-1} is the largest j <= n - 2 with knot[j] <= u
+// (intEV.cu:81-89, a bisection there).  Here nb equal buckets over
+// r = sqrt(u) each hold a record (s, knot[s], knot[s+1], knot[s+2]) with s at
+// or below the interval of any u that can land in the bucket (entries past
+// n - 2 are +inf).  When the host has checked that the interval is always one
+// of s, s+1, s+2 ("fast"), two comparisons give it with no further reads;
+// otherwise a scan from s ends on it.  Either way the result is the
+// bisection's, whatever the rounding of r, and nothing branches on u in the
+// fast case.
+struct SplineRef {
+  const float4* cf;       // coefficient rows (c0, c1, c2, c3)
+  const float4* rec;      // bucket records
+  const float* kn;        // knots rho^2
+  int n, nb, fast;
+  float r0, inv_w, k0, kl;
+};
+
+__device__ __forceinline__ SplineRef spline_ref(const float4* blob, const DevGeom& g) {
+  SplineRef s;
+  s.cf = blob + g.spline_cf;
+  s.rec = blob + g.spline_rec;
+  s.kn = (const float*)blob + g.spline_kn;
+  s.n = g.spline_n;
+  s.nb = g.spline_nb;
+  s.fast = g.spline_fast;
+  s.r0 = g.spline_r0;
+  s.inv_w = g.spline_inv_w;
+  s.k0 = g.spline_k0;
+  s.kl = g.spline_kl;
+  return s;
 }
 
-template <class R>
-__device__ R spline_f(R u, const float* __restrict__ kn, const float4* __restrict__ cf, int n, int mode) {
-  R k0 = (R)kn[0], kl = (R)kn[n - 1];
-  if (u < kl && u > k0) {
-    int j = spline_interval<R>(u, kn, n);
-    float4 c = cf[j];
-    R dx = u - (R)kn[j];
-    return (R)c.x + dx * ((R)c.y + dx * ((R)c.z + dx * (R)c.w));
+// Interval j (0 <= j <= n - 2) of u and its left knot; exact for
+// k0 < u < k_{n-1}, an in-bounds index for any u.  FAST: every table of the
+// launch is fast (no scan code at all).
+template <class R, bool FAST>
+__device__ __forceinline__ int spline_find(R u, const SplineRef& sp, R& klo) {
+  const float r = __builtin_amdgcn_sqrtf((float)u);
+  int b = (int)((r - sp.r0) * sp.inv_w);
+  b = b < 0 ? 0 : (b >= sp.nb ? sp.nb - 1 : b);
+  const float4 q = sp.rec[b];
+  int j = __float_as_int(q.x);
+  if (FAST || sp.fast) {
+    const bool a1 = u >= (R)q.z, a2 = u >= (R)q.w;
+    klo = a2 ? (R)q.w : (a1 ? (R)q.z : (R)q.y);
+    return j + (int)a1 + (int)a2;
   }
-  if (mode == M_REFERENCE) return (R)0;   // intEV.cu:96-99
-  if (u <= k0) return (k0 > (R)0) ? Math<R>::div((R)cf[0].x * u, k0) : (R)cf[0].x;
-  return (R)cf[n - 1].x;
+  klo = (R)q.y;
+  while (j > 0 && u < klo) { --j; klo = (R)sp.kn[j]; }
+  R khi = (R)sp.kn[j + 1];
+  while (j + 2 < sp.n && khi <= u) { ++j; klo = khi; khi = (R)sp.kn[j + 1]; }
+  return j;
 }
 
-template <class R>
-__device__ R spline_fp(R u, const float* __restrict__ kn, const float4* __restrict__ cf, int n, int mode, const GP<R>& g) {
-  if (mode == M_REFERENCE) {   // intEV.cu:389-398: analytic sin^2 form
-    R s = Math<R>::sin(Math<R>::sqrt(u) * (R)3.14159265358979323846);
-    return (s > (R)100) ? (R)0 : Math<R>::exp(-s * s * g.inv_l2) * g.inv_l2;
+// f(u) and, if need_fp, f'(u) from one lookup and one coefficient row: the
+// cubic of the row and its derivative (the outer rows are polynomials of the
+// same form).  Reference mode: f' from the analytic sin^2 form
+// (intEV.cu:389-398).
+template <class R, bool FAST = false>
+__device__ __forceinline__ void spline_eval(R u, const SplineRef& sp, int mode, bool need_fp, const GP<R>& g,
+                                            R& f, R& fp) {
+  R klo;
+  const int j = spline_find<R, FAST>(u, sp, klo);
+  const bool lo = !(u > (R)sp.k0), hi = !(u < (R)sp.kl);
+  const int row = lo ? 0 : (hi ? sp.n : j + 1);
+  klo = lo ? (R)0 : (hi ? (R)sp.kl : klo);
+  const float4 c = sp.cf[row];
+  const R dx = u - klo;
+  f = (R)c.x + dx * ((R)c.y + dx * ((R)c.z + dx * (R)c.w));
+  fp = (R)0;
+  if (need_fp) {
+    if (mode == M_REFERENCE) {
+      R s = Math<R>::sin(Math<R>::sqrt(u) * (R)3.14159265358979323846);
+      fp = (s > (R)100) ? (R)0 : Math<R>::exp(-s * s * g.inv_l2) * g.inv_l2;
+    } else {
+      fp = (R)c.y + dx * ((R)2 * (R)c.z + dx * (R)3 * (R)c.w);
+    }
   }
-  R k0 = (R)kn[0], kl = (R)kn[n - 1];
-  if (u <= k0) return (k0 > (R)0) ? Math<R>::div((R)cf[0].x, k0) : (R)cf[0].y;
-  if (u >= kl) return (R)0;
-  int j = spline_interval<R>(u, kn, n);
-  float4 c = cf[j];
-  R dx = u - (R)kn[j];
-  return (R)c.y + dx * ((R)2 * (R)c.z + dx * (R)3 * (R)c.w);
 }
 
 // Generic profile evaluation: f(u) and f'(u).
 template <class R>
-__device__ void profile_eval(int kind, const GP<R>& g, R u, int mode, bool need_fp,
-                             const float* __restrict__ kn, const float4* __restrict__ cf, int ns,
+__device__ void profile_eval(int kind, const GP<R>& g, R u, int mode, bool need_fp, const SplineRef& sp,
                              R& f, R& fp) {
   switch (kind) {
     case P_STEP:
@@ -263,7 +312,10 @@ __device__ void profile_eval(int kind, const GP<R>& g, R u, int mode, bool need_
       fp = (u < g.l2) ? (R)2 * g.inv_l2 * ((R)1 - u * g.inv_l2) : (R)0;
       break;
     case P_GAUSSIAN: {
-      R e = Math<R>::exp(-u * g.inv_l2);
+      // FP32: u is in the scaled units of the Gaussian kernels, u' = log2(e) rho^2/l2
+      R e;
+      if constexpr (sizeof(R) == 4) e = __builtin_amdgcn_exp2f(-u);
+      else e = Math<R>::exp(-u * g.inv_l2);
       f = (R)1 - e;
       fp = e * g.inv_l2;
     } break;
@@ -272,8 +324,7 @@ __device__ void profile_eval(int kind, const GP<R>& g, R u, int mode, bool need_
       fp = need_fp ? fplperi<R>(g, u) : (R)0;
       break;
     case P_SPLINE:
-      f = spline_f<R>(u, kn, cf, ns, mode);
-      fp = need_fp ? spline_fp<R>(u, kn, cf, ns, mode, g) : (R)0;
+      spline_eval<R>(u, sp, mode, need_fp, g, f, fp);
       break;
     case P_FIXFLUX:
     case P_FIXFLUX_FF:
@@ -287,9 +338,12 @@ __device__ void profile_eval(int kind, const GP<R>& g, R u, int mode, bool need_
 
 // g(u) = f(u)/u for the generic profiles.  Reference mode: plain quotient
 // (the caller has clamped u).  Physical mode: analytic limits at u -> 0.
-template <class R>
-__device__ R generic_g(int kind, const GP<R>& g, R u, int mode, bool need_fp,
-                       const float* __restrict__ kn, const float4* __restrict__ cf, int ns, R& fp) {
+// PROF is the profile when it is fixed at compile time, P_ANY to take it
+// from kind at run time.
+template <class R, int PROF>
+__device__ __forceinline__ R generic_g(int kind_rt, const GP<R>& g, R u, int mode, bool need_fp,
+                                       const SplineRef& sp, R& fp) {
+  const int kind = PROF == P_ANY ? kind_rt : PROF;
   R f;
   if (mode == M_PHYSICAL) {
     switch (kind) {
@@ -304,14 +358,25 @@ __device__ R generic_g(int kind, const GP<R>& g, R u, int mode, bool need_fp,
       case P_QUADRATIC:
         fp = (u < g.l2) ? (R)2 * g.inv_l2 * ((R)1 - u * g.inv_l2) : (R)0;
         return (u < g.l2) ? ((R)2 - u * g.inv_l2) * g.inv_l2 : Math<R>::div((R)1, u);
+      case P_GAUSSIAN:
+        if constexpr (sizeof(R) == 4) {   // scaled units, as in the Gaussian kernels
+          R e;
+          const R gv = gauss_g(u, 0.0, e);
+          fp = e * g.inv_l2;
+          return gv;
+        }
+        break;
       default:
         break;
     }
-    profile_eval<R>(kind, g, u, mode, true, kn, cf, ns, f, fp);
-    if (u > (R)0) return Math<R>::div(f, u);
+    const bool pos = u > (R)0;
+    if (PROF == P_SPLINE) spline_eval<R, true>(u, sp, mode, need_fp || !pos, g, f, fp);
+    else profile_eval<R>(kind, g, u, mode, need_fp || !pos, sp, f, fp);
+    if (pos) return Math<R>::div(f, u);
     return fp;   // f(0) = 0: the limit of f(u)/u is f'(0)
   }
-  profile_eval<R>(kind, g, u, mode, need_fp, kn, cf, ns, f, fp);
+  if (PROF == P_SPLINE) spline_eval<R, true>(u, sp, mode, need_fp, g, f, fp);
+  else profile_eval<R>(kind, g, u, mode, need_fp, sp, f, fp);
   return Math<R>::div(f, u);
 }
 

@@ -110,8 +110,8 @@ def _load():
         "wfl_plan_graph_run": (I, [P, P]),
         "wfl_plan_run_timed": (I, [P, P, P]),
         "wfl_plan_autotune": (I, [P, P, P]),
-        "wfl_plan_config": (I, [P, P, P, P]),
-        "wfl_plan_set_config": (I, [P, I, I, I]),
+        "wfl_plan_config": (I, [P, P, P, P, P]),
+        "wfl_plan_set_config": (I, [P, I, I, I, I]),
         "wfl_plan_read": (I, [P, P, P]),
         "wfl_plan_ngroups": (I, [P]),
         "wfl_plan_gsum_device": (P, [P]),
@@ -314,12 +314,13 @@ class Plan:
         return c.value
 
     def config(self):
-        m, pe, w = C.c_int(), C.c_int(), C.c_int()
-        check(lib.wfl_plan_config(self.h, C.byref(m), C.byref(pe), C.byref(w)))
-        return m.value, pe.value, w.value
+        """(mapping, persistent, wavefronts per block, tasks per loop group)"""
+        m, pe, w, c = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib.wfl_plan_config(self.h, C.byref(m), C.byref(pe), C.byref(w), C.byref(c)))
+        return m.value, pe.value, w.value, c.value
 
-    def set_config(self, mapping, persistent, wpb):
-        check(lib.wfl_plan_set_config(self.h, mapping, persistent, wpb))
+    def set_config(self, mapping, persistent, wpb, chunks=1):
+        check(lib.wfl_plan_set_config(self.h, mapping, persistent, wpb, chunks))
         self.mapping = mapping
 
     def graph_run(self, stream=None):

@@ -342,6 +342,7 @@ def run_config(wfl, args, mode, cfg):
         out["graph_ms_per_step"] = (time.perf_counter() - t0) / args.steps * 1e3
     gs_m, _ = mixed.read()
     out["nonfinite_loops_mixed"] = mixed.nonfinite()
+    out["accurate_node_tuples"] = mixed.accurate_tuples
     mixed.close()
     fl = algorithmic_flops(cfg)
     out.update({"value": pe / ms, "ms_per_step": ms * 1e3, "kernel_ms": km * 1e3,
@@ -377,6 +378,7 @@ def bench_c2(wfl, args, mode):
     pe_step = point_evals(cfg)
     mixed = r.plan(wfl.MIXED)
     mapping_used = mixed.mapping
+    accurate = mixed.accurate_tuples
     ms_step, k_mixed = r.timed(mixed, args.steps, args.warmup, "mixed")
     gs_mixed, _ = mixed.read()
     nonfinite = mixed.nonfinite()
@@ -410,6 +412,7 @@ def bench_c2(wfl, args, mode):
         with open(tr) as fh:
             result["roofline"]["traffic"] = json.load(fh).get("hbm_bytes_per_launch")
     result["nonfinite_loops"] = nonfinite
+    result["accurate_node_tuples"] = accurate
     if not args.no_fp64:
         fp64 = r.plan(wfl.FP64)
         ms64, k64 = r.timed(fp64, args.fp64_steps, 0, "fp64")
@@ -419,6 +422,9 @@ def bench_c2(wfl, args, mode):
         result["mixed_over_fp64"] = result["value"] / (pe_step / ms64)
         result["fp64_kernel_ms"] = k64 * 1e3
         result["accuracy_mixed_vs_fp64"] = accuracy(wfl, cfg, gs_mixed, gs64)
+        if args.dump:
+            np.savez_compressed(args.dump, mixed=gs_mixed.sum(axis=1), fp64=gs64.sum(axis=1),
+                                tup=np.stack(cfg["tup"]))
     if not args.no_ref_mode and mode == wfl.MODE_PHYSICAL:
         ref = r.plan(wfl.MIXED, wfl.MODE_REFERENCE)
         msr, _ = r.timed(ref, 1, 0, "mixed-reference-mode")
@@ -447,6 +453,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-ref-mode", action="store_true")
     ap.add_argument("--only-mixed", action="store_true", help="time the mixed engine only (profiling runs)")
+    ap.add_argument("--dump", default="", metavar="NPZ",
+                    help="save the per-tuple sums over groups of W - 1 (mixed and FP64) of the c2 run")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()

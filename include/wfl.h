@@ -150,6 +150,10 @@ int wfl_plan_graph_run(wfl_plan *p, void *stream);
  * (W_N - 1, W_{N/2} - 1, W_{N/4} - 1); perloop [ntup][Nl][6] if kept. */
 int wfl_plan_read(wfl_plan *p, double *gsum, double *perloop);
 int wfl_plan_ngroups(const wfl_plan *p);
+/* Tuples routed to the accurate FP32 Gaussian node (mixed plans): their loops
+ * are small (sigma sqrt(T) < 1/2, sigma^2 = log2(e)/l2) and can reach the tube
+ * core, where the plain quotient (1 - 2^-u)/u loses relative accuracy. */
+int wfl_plan_accurate_tuples(const wfl_plan *p);
 /* Device pointer of the group sums (for callers that keep results on device). */
 const double *wfl_plan_gsum_device(const wfl_plan *p);
 /* Number of non-finite per-loop values met in the last run (diagnostic). */

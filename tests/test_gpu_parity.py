@@ -195,3 +195,22 @@ def test_other_profiles_mixed(gpu, orc, loops, kind):
     e1, e2 = relerr(pl, rpl), relerr(gs, rgs)
     print(f"  {kind}: perloop {e1:.2e} gsum {e2:.2e}")
     assert e1 < 1e-4 and e2 < 1e-4
+
+
+@pytest.mark.parametrize("spinor", [0, 1])
+def test_gaussian_mixed_small_loops_in_core(gpu, orc, loops, spinor):
+    """Loops far smaller than the tube and inside its core (lambda = 2,
+    T = 1e-3 .. 1e-2 at and near the axis): the plain FP32 quotient
+    (1 - 2^-u)/u would carry a relative error ~6e-8/u per node, ~1e-4 in the
+    phase here; the accurate nodes keep the per-loop phases within 2e-6 of
+    the oracle (relative to the largest) and the spin sums within 1e-6."""
+    geoms = [gpu.Geometry(kind=gpu.GAUSSIAN, F=1.0, l2=4.0)]
+    og = [orc.Geom(orc.GAUSSIAN, F=1.0, l2=4.0)]
+    gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, 1, 2, spinor, gpu.MIXED, 1,
+                                tuples([0.0, 0.05, 0.3], [1e-3, 3e-3, 1e-2]))
+    for t in range(pl.shape[0]):
+        e_phase = relerr(pl[t, :, :3], rpl[t, :, :3])
+        assert e_phase < 2e-6, (t, e_phase)
+        if spinor:
+            assert relerr(pl[t, :, 3:], rpl[t, :, 3:]) < 1e-6
+    print(f"  max phase err {max(relerr(pl[t, :, :3], rpl[t, :, :3]) for t in range(pl.shape[0])):.2e}")

@@ -91,6 +91,23 @@ __global__ void k_records(const float* __restrict__ x, const float* __restrict__
   r[7] = (R)0;
 }
 
+// Largest |w|^2 over all points: block maximum, then one atomic per block
+// on the float's bits (non-negative floats order like their bit patterns).
+__global__ __launch_bounds__(256) void k_max_radius2(const float* __restrict__ x, const float* __restrict__ y,
+                                                     long long n, unsigned int* __restrict__ out) {
+  __shared__ float red[256];
+  float m = 0.0f;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    m = fmaxf(m, fmaf(x[i], x[i], y[i] * y[i]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + k]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out, __float_as_uint(red[0]));
+}
+
 __global__ void k_aos_to_soa(const float4* __restrict__ aos, long long n, float* __restrict__ x, float* __restrict__ y) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -172,6 +189,7 @@ struct wfl_loops_s {
   float* y = nullptr;
   float* rec32 = nullptr;  // [Nl][N][REC]
   double* rec64 = nullptr;
+  float rmax = -1.0f;      // largest |w| over the ensemble (< 0: not yet measured)
 };
 
 static int loops_alloc(wfl_loops** out, int Nl, int N) {
@@ -257,6 +275,30 @@ extern "C" int wfl_loops_shape(const wfl_loops* L, int* Nl, int* N) {
   if (!L) { wfl_set_error("wfl_loops_shape: NULL"); return WFL_ERR_ARG; }
   if (Nl) *Nl = L->Nl;
   if (N) *N = L->N;
+  return WFL_OK;
+}
+
+// The ensemble's largest loop radius max |w| (measured once, on the device).
+static int loops_rmax(wfl_loops* L, float* rmax) {
+  if (L->rmax < 0.0f) {
+    unsigned int* d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(unsigned int)));
+    unsigned int h = 0;
+    hipError_t e = hipMemcpy(d, &h, sizeof h, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      const long long n = (long long)L->Nl * L->N;
+      const unsigned grid = (unsigned)std::min<long long>((n + 255) / 256, 4096);
+      hipLaunchKernelGGL(k_max_radius2, dim3(grid), dim3(256), 0, 0, L->x, L->y, n, d);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof h, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) { wfl_set_error("loop radius: %s", hipGetErrorString(e)); return WFL_ERR_DEVICE; }
+    float r2;
+    memcpy(&r2, &h, sizeof r2);
+    L->rmax = sqrtf(r2) * (1.0f + 1e-6f);
+  }
+  *rmax = L->rmax;
   return WFL_OK;
 }
 
@@ -452,6 +494,29 @@ static void fill_tuple(double sg, const DevGeom& dg, int gi, double cx, double c
   }
 }
 
+// Accurate FP32 Gaussian nodes for a tuple (scaled units, sigma != 1): the
+// loops are small (s' = sigma sqrt(T) < 1/2) and some point can come within
+// |z'| < 0.75 (u' < 0.56) of the nearest tube axis, given the ensemble's
+// largest radius rmax.  The plain quotient's phase error for a loop in the
+// core falls roughly as 1/s'^2: ~4e-7 relative in W - 1 at s' = 0.19, ~1e-8
+// at s' = 0.6 (5120 loops x 4096 points, lambda = 2); from s' = 1/2 up it is
+// below 1e-7, and loops away from the core have no nodes there.
+static int tuple_needs_accurate(const DevGeom& dg, const DevTuple& d, double sg, float rmax) {
+  if (dg.kind != WFL_GAUSSIAN || sg == 1.0) return 0;
+  const double s = d.s;
+  if (!(s < 0.5)) return 0;
+  double dx = d.cx, dy = d.cy;
+  if (dg.lattice > 0.0) {
+    const double L = dg.latticef;
+    dx -= L * floor(dx / L + 0.5);
+    dy -= L * floor(dy / L + 0.5);
+  }
+  if (!(sqrt(dx * dx + dy * dy) - s * (double)rmax < 0.75)) return 0;
+  // isolated tube: every point is within 0.75 + 2 s' rmax < 10 of the axis,
+  // so u' < 100 and the bounded accurate node applies; else the clamped one
+  return (dg.lattice <= 0.0 && 0.75 + 2.0 * s * (double)rmax < 9.5) ? 1 : 2;
+}
+
 extern "C" int wfl_effective_tuple(const wfl_geometry* g, double cx, double cy, double T, double out[6]) {
   if (!g || !out || !(g->l2 > 0.0) || !(T > 0.0)) { wfl_set_error("wfl_effective_tuple: bad arguments"); return WFL_ERR_ARG; }
   DevGeom dg;
@@ -610,6 +675,11 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
   }
   int rc0 = require_device();
   if (rc0) return rc0;
+  float rmax = 0.0f;
+  if (b->precision == WFL_MIXED) {
+    rc0 = loops_rmax(L, &rmax);
+    if (rc0) return rc0;
+  }
 
   wfl_plan* p = new wfl_plan();
   p->b = *b;
@@ -648,19 +718,22 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
     const int gi = b->tup_geom[t];
     int nl = Nl;
     if (b->geom_nloops) nl = std::max(0, std::min(Nl, b->geom_nloops[gi]));
-    fill_tuple(unit_scale(b->geoms[gi], true), geoms[gi], gi, b->tup_cx[t], b->tup_cy[t], sqrt(b->tup_T[t]), nl,
-               p->tups[t]);
+    const double sg = unit_scale(b->geoms[gi], true);
+    fill_tuple(sg, geoms[gi], gi, b->tup_cx[t], b->tup_cy[t], sqrt(b->tup_T[t]), nl, p->tups[t]);
+    if (b->precision == WFL_MIXED) p->tups[t].acc = tuple_needs_accurate(geoms[gi], p->tups[t], sg, rmax);
   }
-  // lane groups: tuples sorted by geometry, each geometry padded to 64 lanes
+  // lane groups: tuples sorted by geometry and node kind, each such set
+  // padded to 64 lanes (one geometry and one node kind per wavefront)
   std::vector<int> order(b->ntup);
   for (int t = 0; t < b->ntup; ++t) order[t] = t;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return p->tups[a].geom < p->tups[c].geom; });
+  auto key = [&](int t) { return 4LL * p->tups[t].geom + p->tups[t].acc; };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return key(a) < key(c); });
   std::vector<int> lanes;
   std::vector<int> lg_nloops;
   for (size_t k = 0; k < order.size();) {
-    int g = p->tups[order[k]].geom;
+    const long long g = key(order[k]);
     size_t e = k;
-    while (e < order.size() && p->tups[order[e]].geom == g) ++e;
+    while (e < order.size() && key(order[e]) == g) ++e;
     for (size_t s = k; s < e; s += 64) {
       int nl = 0;
       for (size_t q = 0; q < 64; ++q) {
@@ -963,6 +1036,12 @@ extern "C" int wfl_plan_read(wfl_plan* p, double* gsum, double* perloop) {
 }
 
 extern "C" int wfl_plan_ngroups(const wfl_plan* p) { return p ? p->ngroups : 0; }
+extern "C" int wfl_plan_accurate_tuples(const wfl_plan* p) {
+  if (!p) return 0;
+  int n = 0;
+  for (const DevTuple& t : p->tups) n += t.acc ? 1 : 0;
+  return n;
+}
 extern "C" const double* wfl_plan_gsum_device(const wfl_plan* p) { return p ? p->d_gsum : nullptr; }
 extern "C" int wfl_plan_mapping(const wfl_plan* p) { return p ? p->mapping : 0; }
 
@@ -1058,6 +1137,12 @@ extern "C" int wfl_internal_ev_launch(const void* dev_aos, int Nl, int N, const 
   fill_geom(gg, g->kind == WFL_SPLINE ? &sb : nullptr, scaled, dg);
   DevTuple dt;
   fill_tuple(unit_scale(gg, scaled), dg, 0, (double)cx, (double)cy, (double)rtT, Nl, dt);
+  if (scaled) {
+    float rmax = 0.0f;
+    rc = loops_rmax(g_ev.L, &rmax);
+    if (rc) return rc;
+    dt.acc = tuple_needs_accurate(dg, dt, unit_scale(gg, scaled), rmax);
+  }
   HIP_TRY(hipMemcpy(g_ev.d_geom, &dg, sizeof dg, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(g_ev.d_tup, &dt, sizeof dt, hipMemcpyHostToDevice));
   WflLaunch a;

@@ -58,12 +58,12 @@ template <class R> struct LaneC {
 };
 
 // --------------------------------------------------------- node evaluation
-template <class R, int PROF>
+template <class R, int PROF, int ACC>
 __device__ __forceinline__ R node_g(R u, const LaneC<R>& lc, int kind, const GP<R>& gp, int mode,
                                     const SplineRef& sp, bool spinor, R& fp) {
   if (PROF == P_GAUSSIAN) {
     R e;
-    R gval = gauss_g(u, (double)lc.inv_l2, e);
+    R gval = gauss_g<ACC>(u, (double)lc.inv_l2, e);
     fp = e;
     return gval;
   } else {
@@ -76,7 +76,7 @@ __device__ __forceinline__ R node_g(R u, const LaneC<R>& lc, int kind, const GP<
 // midpoint weight.  Adds to P (phase) and S (half the spin sum) of the
 // point's class, before the common factors F s h/3 (phase) and
 // 2 h/3 spin_scale (spin), h = 1/n.
-template <class R, int PROF, int SPINOR, int MODE, int NSUB>
+template <class R, int PROF, int SPINOR, int MODE, int NSUB, int ACC>
 __device__ __forceinline__ void point_eval(R wx, R wy, R dx4, R dy4, R ex, R ey,
                                            R ccx, R ccy, const LaneC<R>& lc, int kind, const GP<R>& gp,
                                            const SplineRef& sp, int nsub_rt, R& P, R& S) {
@@ -89,12 +89,12 @@ __device__ __forceinline__ void point_eval(R wx, R wy, R dx4, R dy4, R ex, R ey,
     R E = fma(zx, ey, -(zy * ex));        // z x e: both endpoint weights of this point
     R A = fma(zx, zx, fma(zy, zy, tiny));
     R fA, fk;
-    R gA = node_g<R, PROF>(A, lc, kind, gp, MODE, sp, SPINOR, fA);
+    R gA = node_g<R, PROF, ACC>(A, lc, kind, gp, MODE, sp, SPINOR, fA);
     R gin, fin;
     if (n == 2) {
       R mx = fma(lc.m8, dx4, zx), my = fma(lc.m8, dy4, zy);
       R Am = fma(mx, mx, fma(my, my, tiny));
-      gin = node_g<R, PROF>(Am, lc, kind, gp, MODE, sp, SPINOR, fk);
+      gin = node_g<R, PROF, ACC>(Am, lc, kind, gp, MODE, sp, SPINOR, fk);
       fin = fk + fk;
     } else {
       const R h4 = lc.s4 / (R)n;          // node k sits at z + (k h s/4)(4 dw)
@@ -103,7 +103,7 @@ __device__ __forceinline__ void point_eval(R wx, R wy, R dx4, R dy4, R ex, R ey,
         R t = (R)k * h4;
         R px = fma(t, dx4, zx), py = fma(t, dy4, zy);
         R u = fma(px, px, fma(py, py, tiny));
-        R gk = node_g<R, PROF>(u, lc, kind, gp, MODE, sp, SPINOR, fk);
+        R gk = node_g<R, PROF, ACC>(u, lc, kind, gp, MODE, sp, SPINOR, fk);
         gin = (k & 1) ? gin + gk : fma((R)0.5, gk, gin);      // Simpson weights 4, 2 over 4
         if (SPINOR) fin = (k & 1) ? fma((R)2, fk, fin) : fin + fk;   // and over 2
       }
@@ -122,13 +122,13 @@ __device__ __forceinline__ void point_eval(R wx, R wy, R dx4, R dy4, R ex, R ey,
     R A1 = A + (R)2 * B + C;
     A1 = A1 < lc.c8 ? lc.c8 : A1;
     R f0, f1, fk;
-    R g0 = node_g<R, PROF>(A, lc, kind, gp, MODE, sp, SPINOR, f0);
-    R g1 = node_g<R, PROF>(A1, lc, kind, gp, MODE, sp, SPINOR, f1);
+    R g0 = node_g<R, PROF, ACC>(A, lc, kind, gp, MODE, sp, SPINOR, f0);
+    R g1 = node_g<R, PROF, ACC>(A1, lc, kind, gp, MODE, sp, SPINOR, f1);
     R gs = (R)0.25 * (g0 + g1), fs = SPINOR ? (R)0.5 * (f0 + f1) : (R)0;
     if (n == 2) {
       R u = A + B + (R)0.25 * C;
       u = u < lc.c10 ? lc.c10 : u;
-      gs += node_g<R, PROF>(u, lc, kind, gp, MODE, sp, SPINOR, fk);
+      gs += node_g<R, PROF, ACC>(u, lc, kind, gp, MODE, sp, SPINOR, fk);
       if (SPINOR) fs = fma((R)2, fk, fs);
     } else {
       const R h = (R)1 / (R)n;
@@ -136,7 +136,7 @@ __device__ __forceinline__ void point_eval(R wx, R wy, R dx4, R dy4, R ex, R ey,
         R t = (R)k * h;
         R u = A + (R)2 * B * t + C * t * t;
         u = u < lc.c10 ? lc.c10 : u;
-        R gk = node_g<R, PROF>(u, lc, kind, gp, MODE, sp, SPINOR, fk);
+        R gk = node_g<R, PROF, ACC>(u, lc, kind, gp, MODE, sp, SPINOR, fk);
         gs = (k & 1) ? gs + gk : fma((R)0.5, gk, gs);
         if (SPINOR) fs = (k & 1) ? fma((R)2, fk, fs) : fs + fk;
       }
@@ -225,6 +225,41 @@ __device__ __forceinline__ const float4* stage_splines(const float4* __restrict_
 }
 
 // ------------------------------------------------------ lane per tuple
+// Class sums of one loop over all image tubes for one lane's tuple: the
+// points in 32-point blocks (FP32 partial sums, added to the FP64 P and S
+// after each block), four points per step (wave-uniform records).  ACC
+// selects the FP32 Gaussian node (0 plain, 1 and 2 accurate).
+template <class R, int PROF, int SPINOR, int MODE, int NSUB, int ACC>
+__device__ __forceinline__ void loop_sums(const R* __restrict__ rl, int N, int M, bool lattice, const DevTuple& tp,
+                                          R cx, R cy, R lat, const LaneC<R>& lc, int kind, const GP<R>& gp,
+                                          const SplineRef& sp, int nsub_rt, double P[3], double S[3]) {
+  for (int ia = -M; ia <= M; ++ia) {
+    for (int ib = -M; ib <= M; ++ib) {
+      R ccx = cx, ccy = cy;
+      if (M > 0 || lattice) {
+        ccx = fma(-(R)(tp.m0 + ia), lat, cx);
+        ccy = fma(-(R)(tp.n0 + ib), lat, cy);
+      }
+      for (int i0 = 0; i0 < N; i0 += 32) {
+        R p[3] = {(R)0, (R)0, (R)0}, q[3] = {(R)0, (R)0, (R)0};
+        const int i1 = i0 + 32 < N ? i0 + 32 : N;
+        for (int i = i0; i < i1; i += 4) {
+          const R* r = rl + (size_t)i * REC;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int c = (u & 1) ? 0 : (u == 0 ? 2 : 1);
+            point_eval<R, PROF, SPINOR, MODE, NSUB, ACC>(r[REC * u + 0], r[REC * u + 1], r[REC * u + 2],
+                                                         r[REC * u + 3], r[REC * u + 4], r[REC * u + 5], ccx, ccy,
+                                                         lc, kind, gp, sp, nsub_rt, p[c], q[c]);
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { P[c] += (double)p[c]; if (SPINOR) S[c] += (double)q[c]; }
+      }
+    }
+  }
+}
+
 // SPLIT = 1 writes each loop's (Wsscal, Wsferm) to perloop and leaves W - 1
 // and the group sums to k_finalize_groups: the FP64 trigonometry then does
 // not set this kernel's register allocation (more resident wavefronts).
@@ -259,35 +294,20 @@ __device__ __forceinline__ void tuple_task(
   const R lat = sizeof(R) == 4 ? (R)gm.latticef : (R)gm.lattice;
   const R cx = sizeof(R) == 4 ? (R)tp.cx : (R)tp.cxd, cy = sizeof(R) == 4 ? (R)tp.cy : (R)tp.cyd;
 
+  // accurate Gaussian nodes: uniform over the lane group (the host groups them)
+  const int acc = (sizeof(R) == 4 && PROF == P_GAUSSIAN) ? __builtin_amdgcn_readfirstlane(tp.acc) : 0;
+  const bool lattice = gm.lattice > 0.0;
+
   double G[3] = {0.0, 0.0, 0.0};
   for (int l = loop0; l < loop0 + nloop; ++l) {
     const R* __restrict__ rl = (const R*)__builtin_assume_aligned(rec + (size_t)l * N * REC, REC * sizeof(R));
     double P[3] = {0.0, 0.0, 0.0}, S[3] = {0.0, 0.0, 0.0};
-    for (int ia = -M; ia <= M; ++ia) {
-      for (int ib = -M; ib <= M; ++ib) {
-        R ccx = cx, ccy = cy;
-        if (M > 0 || gm.lattice > 0.0) {
-          ccx = fma(-(R)(tp.m0 + ia), lat, cx);
-          ccy = fma(-(R)(tp.n0 + ib), lat, cy);
-        }
-        for (int i0 = 0; i0 < N; i0 += 32) {
-          R p[3] = {(R)0, (R)0, (R)0}, q[3] = {(R)0, (R)0, (R)0};
-          const int i1 = i0 + 32 < N ? i0 + 32 : N;
-          for (int i = i0; i < i1; i += 4) {
-            const R* r = rl + (size_t)i * REC;
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const int c = (u & 1) ? 0 : (u == 0 ? 2 : 1);
-              point_eval<R, PROF, SPINOR, MODE, NSUB>(r[REC * u + 0], r[REC * u + 1], r[REC * u + 2], r[REC * u + 3],
-                                                      r[REC * u + 4], r[REC * u + 5], ccx, ccy, lc, kind, gp, sp,
-                                                      nsub_rt, p[c], q[c]);
-            }
-          }
-#pragma unroll
-          for (int c = 0; c < 3; ++c) { P[c] += (double)p[c]; if (SPINOR) S[c] += (double)q[c]; }
-        }
-      }
-    }
+    if (acc == 1)
+      loop_sums<R, PROF, SPINOR, MODE, NSUB, 1>(rl, N, M, lattice, tp, cx, cy, lat, lc, kind, gp, sp, nsub_rt, P, S);
+    else if (acc == 2)
+      loop_sums<R, PROF, SPINOR, MODE, NSUB, 2>(rl, N, M, lattice, tp, cx, cy, lat, lc, kind, gp, sp, nsub_rt, P, S);
+    else
+      loop_sums<R, PROF, SPINOR, MODE, NSUB, 0>(rl, N, M, lattice, tp, cx, cy, lat, lc, kind, gp, sp, nsub_rt, P, S);
     double sc[3], fe[3];
     finalize_loop(P, S, gm.F, s_phase, tp.sd * tp.sd, spin_scale, n, N, SPINOR, sc, fe);
     if (!SPLIT) {
@@ -320,14 +340,16 @@ __device__ __forceinline__ int next_task(int* __restrict__ queue, int lane) {
 // Persistent: a grid sized to the machine; each wavefront keeps taking the
 // next task from *queue (zeroed before the launch) until none is left, so
 // uneven tasks (lattice images, partial loop groups) balance themselves.
+#define WFL_K_TUPLE_PARAMS                                                                   \
+    const R* __restrict__ rec, int N, int Nl, const DevTuple* __restrict__ tups,             \
+    const DevGeom* __restrict__ geoms, const int* __restrict__ lanes,                        \
+    const float4* __restrict__ sblob, int sblob_f4,                                          \
+    const Task* __restrict__ tasks, int ntasks, int nslots, int nsub_rt,                     \
+    double* __restrict__ gsum, double* __restrict__ perloop,                                 \
+    unsigned long long* __restrict__ nonfinite, int wpb, int persistent, int* __restrict__ queue
+
 template <class R, int PROF, int SPINOR, int MODE, int NSUB, int SPLIT>
-__global__ __launch_bounds__(256) void k_tuple(
-    const R* __restrict__ rec, int N, int Nl, const DevTuple* __restrict__ tups,
-    const DevGeom* __restrict__ geoms, const int* __restrict__ lanes,
-    const float4* __restrict__ sblob, int sblob_f4,
-    const Task* __restrict__ tasks, int ntasks, int nslots, int nsub_rt,
-    double* __restrict__ gsum, double* __restrict__ perloop,
-    unsigned long long* __restrict__ nonfinite, int wpb, int persistent, int* __restrict__ queue) {
+__device__ __forceinline__ void k_tuple_body(WFL_K_TUPLE_PARAMS) {
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const float4* sb = stage_splines<PROF>(sblob, sblob_f4);
@@ -340,7 +362,42 @@ __global__ __launch_bounds__(256) void k_tuple(
   }
 }
 
+template <class R, int PROF, int SPINOR, int MODE, int NSUB, int SPLIT>
+__global__ __launch_bounds__(256) void k_tuple(WFL_K_TUPLE_PARAMS) {
+  k_tuple_body<R, PROF, SPINOR, MODE, NSUB, SPLIT>(rec, N, Nl, tups, geoms, lanes, sblob, sblob_f4, tasks, ntasks,
+                                                   nslots, nsub_rt, gsum, perloop, nonfinite, wpb, persistent, queue);
+}
+
 // ------------------------------------------------------- wave per loop
+// One loop's sums for one tuple, lanes striding over points (lane l takes
+// points l + 64k, all of class l & 3), in blocks of 8 points per lane.
+template <class R, int PROF, int SPINOR, int MODE, int NSUB, int ACC>
+__device__ __forceinline__ void wave_loop_sums(const R* __restrict__ rl, int N, int M, bool lattice,
+                                               const DevTuple& tp, R cx, R cy, R lat, const LaneC<R>& lc, int kind,
+                                               const GP<R>& gp, const SplineRef& sp, int nsub_rt, int lane,
+                                               double& P, double& S) {
+  for (int ia = -M; ia <= M; ++ia) {
+    for (int ib = -M; ib <= M; ++ib) {
+      R ccx = cx, ccy = cy;
+      if (lattice) {
+        ccx = fma(-(R)(tp.m0 + ia), lat, cx);
+        ccy = fma(-(R)(tp.n0 + ib), lat, cy);
+      }
+      for (int i0 = lane; i0 < N; i0 += 64 * 8) {
+        R p = (R)0, q = (R)0;
+        const int i1 = i0 + 64 * 8 < N ? i0 + 64 * 8 : N;
+        for (int i = i0; i < i1; i += 64) {
+          const R* r = rl + (size_t)i * REC;
+          point_eval<R, PROF, SPINOR, MODE, NSUB, ACC>(r[0], r[1], r[2], r[3], r[4], r[5], ccx, ccy, lc, kind, gp,
+                                                       sp, nsub_rt, p, q);
+        }
+        P += (double)p;
+        S += (double)q;
+      }
+    }
+  }
+}
+
 __device__ __forceinline__ double shfl_xor_d(double v, int m) {
   return __shfl_xor(v, m, 64);
 }
@@ -376,26 +433,16 @@ __global__ __launch_bounds__(256) void k_loop(
   const R* __restrict__ rl = rec + (size_t)l * N * REC;
 
   double P = 0.0, S = 0.0;      // this lane's class: lane & 3 (points i = lane + 64k)
-  for (int ia = -M; ia <= M; ++ia) {
-    for (int ib = -M; ib <= M; ++ib) {
-      R ccx = cx, ccy = cy;
-      if (gm.lattice > 0.0) {
-        ccx = fma(-(R)(tp.m0 + ia), lat, cx);
-        ccy = fma(-(R)(tp.n0 + ib), lat, cy);
-      }
-      for (int i0 = lane; i0 < N; i0 += 64 * 8) {
-        R p = (R)0, q = (R)0;
-        const int i1 = i0 + 64 * 8 < N ? i0 + 64 * 8 : N;
-        for (int i = i0; i < i1; i += 64) {
-          const R* r = rl + (size_t)i * REC;
-          point_eval<R, PROF, SPINOR, MODE, NSUB>(r[0], r[1], r[2], r[3], r[4], r[5], ccx, ccy, lc, kind, gp, sp,
-                                                  nsub_rt, p, q);
-        }
-        P += (double)p;
-        S += (double)q;
-      }
-    }
-  }
+  const int acc = (sizeof(R) == 4 && PROF == P_GAUSSIAN) ? tp.acc : 0;
+  if (acc == 1)
+    wave_loop_sums<R, PROF, SPINOR, MODE, NSUB, 1>(rl, N, M, gm.lattice > 0.0, tp, cx, cy, lat, lc, kind, gp, sp,
+                                                   nsub_rt, lane, P, S);
+  else if (acc == 2)
+    wave_loop_sums<R, PROF, SPINOR, MODE, NSUB, 2>(rl, N, M, gm.lattice > 0.0, tp, cx, cy, lat, lc, kind, gp, sp,
+                                                   nsub_rt, lane, P, S);
+  else
+    wave_loop_sums<R, PROF, SPINOR, MODE, NSUB, 0>(rl, N, M, gm.lattice > 0.0, tp, cx, cy, lat, lc, kind, gp, sp,
+                                                   nsub_rt, lane, P, S);
   // lanes with equal (lane & 3) hold the same class: reduce over them
 #pragma unroll
   for (int m = 4; m < 64; m <<= 1) {

@@ -43,7 +43,8 @@ struct DevGeom {
 // host from the effective doubles.
 struct DevTuple {
   int geom, m0, n0, nloops;
-  float cx, cy, s, padf;        // FP32 kernel units
+  float cx, cy, s;              // FP32 kernel units
+  int acc;                      // FP32 Gaussian node: 0 plain, 1 accurate (u' < 100), 2 accurate (any u')
   double cxd, cyd, sd, padd;    // effective values (FP64 kernels)
 };
 
@@ -84,18 +85,41 @@ template <> struct GP<double> {
 
 // -------------------------------------------- Gaussian tube (hot profile)
 // FP32, scaled units (u' = log2(e) rho^2 / l2): returns g' = (1 - e)/u' and
-// e = 2^-u' = exp(-rho^2/l2).  Near the tube axis 1 - e carries the absolute
-// rounding of e, a relative error ~6e-8/u'; the phase weight of such a node
-// is z x dw, which shrinks with |z|, so its phase error stays below
-// ~F |dw| 6e-8 / |z| and is random in sign.  The caller adds 1e-30 to u'
-// so that a node exactly on the axis (where z x dw = 0) gives 0, not NaN.
+// e = 2^-u' = exp(-rho^2/l2).  The caller adds 1e-30 to u'.
+//
+// ACC = 0: the plain quotient.  For small u' the difference 1 - e carries
+// the absolute rounding of e, a relative error ~6e-8/u' in the quotient; a
+// loop of scaled size s' lying in the tube core then gets a relative phase
+// error that grows like 1/s'^2.
+// ACC = 1, 2: the argument v~ = -log2(e) is the one whose power of two e is
+// exactly, and 1 - e is exact (Sterbenz), so (1 - e)/v~ is g'(v~), which is
+// off from g'(u') by O(|v~ - u'|) = O(1e-7) absolutely: ~1e-7 relative at
+// every u' (the device of Kahan's expm1 from exp and log).  u' is held at or
+// above 1e-6 in the exponential (g' there is ln 2 to 3.5e-7; e for such a
+// node, within 1e-3 of the axis, is off by < 7e-7).  ACC = 1 needs u' < 100
+// (e stays a normal number), which the host guarantees for the tuples it
+// routes there; ACC = 2 also clamps u' at 100 in the exponential and adds
+// u' - 100 back to the denominator (g' = 1/u' there to 2^-100).
+template <int ACC>
 __device__ __forceinline__ float gauss_g(float u, double, float& e) {
-  e = __builtin_amdgcn_exp2f(-u);
-  const float r = __builtin_amdgcn_rcpf(u);
-  return fmaf(-e, r, r);          // (1 - e)/u in one FMA
+  if constexpr (ACC == 0) {
+    e = __builtin_amdgcn_exp2f(-u);
+    const float r = __builtin_amdgcn_rcpf(u);
+    return fmaf(-e, r, r);          // (1 - e)/u in one FMA
+  } else if constexpr (ACC == 1) {
+    e = __builtin_amdgcn_exp2f(-fmaxf(u, 1e-6f));
+    const float vt = -__builtin_amdgcn_logf(e);
+    return (1.0f - e) * __builtin_amdgcn_rcpf(vt);
+  } else {
+    const float v = __builtin_amdgcn_fmed3f(u, 1e-6f, 100.0f);
+    e = __builtin_amdgcn_exp2f(-v);
+    const float d = fmaxf(u - v, 0.0f) - __builtin_amdgcn_logf(e);
+    return (1.0f - e) * __builtin_amdgcn_rcpf(d);
+  }
 }
 // FP64, problem units: g(u) = -expm1(-u/l2)/u (limit 1/l2 at u = 0),
 // e = 1 + expm1(-u/l2).
+template <int ACC>
 __device__ __forceinline__ double gauss_g(double u, double inv_l2, double& e) {
   double x = u * inv_l2;
   double em1 = ::expm1(-x);
@@ -361,7 +385,7 @@ __device__ __forceinline__ R generic_g(int kind_rt, const GP<R>& g, R u, int mod
       case P_GAUSSIAN:
         if constexpr (sizeof(R) == 4) {   // scaled units, as in the Gaussian kernels
           R e;
-          const R gv = gauss_g(u, 0.0, e);
+          const R gv = gauss_g<2>(u, 0.0, e);
           fp = e * g.inv_l2;
           return gv;
         }

@@ -56,6 +56,8 @@ for _n, _r, _a in [
     ("orc_magfield", _D, [_P, _D]),
     ("orc_smallT", _D, [_P, _D, _D]),
     ("orc_igrand", _D, [_P, _I, _D, _D, _D, _D, _P]),
+    ("orc_field_at", _D, [_P, _D, _D, _P]),
+    ("orc_igrand_B", _D, [_P, _I, _D, _D, _D, _D, _P]),
     ("orc_spline", None, [_P, _P, _I, _I, _P, _P]),
     ("orc_logsimpson", _D, [_P, _P, _I]),
 ]:
@@ -204,10 +206,18 @@ def logsimpson(T, f):
     return lib.orc_logsimpson(_p(T), _p(f), len(T))
 
 
+def field_at(geom, cx, cy):
+    """(eB at x_cm over the lattice image window, distance to the nearest tube)."""
+    r = C.c_double(0.0)
+    B = lib.orc_field_at(geom.ref, cx, cy, C.byref(r))
+    return B, r.value
+
+
 def leff_groups(geom, fermion, T, cx, cy, gsum, counts, policy=1):
     """Oracle L_eff per group: Tfunc's branch policy at each node's effective
     parameters, composite Simpson in ln T over the nominal nodes, /(-2pi)
-    scalar or /(4pi) spinor (Tint.c:69-72)."""
+    scalar or /(4pi) spinor (Tint.c:69-72).  The field is the image-window
+    sum for lattice geometries."""
     ng = gsum.shape[1]
     out = np.zeros(ng)
     f = np.zeros(len(T))
@@ -215,12 +225,12 @@ def leff_groups(geom, fermion, T, cx, cy, gsum, counts, policy=1):
         for k, Tn in enumerate(T):
             ex, ey, rt = geom.tuple(cx, cy, Tn)
             t = rt * rt
-            rho2 = ex * ex + ey * ey
+            B, rho = field_at(geom, ex, ey)
             g3 = gsum[k, q]
             evm1 = (8.0 / 3.0 * g3[0] - 2.0 * g3[1] + 1.0 / 3.0 * g3[2]) / counts[q]
-            v, br = igrand(geom, fermion, t, np.sqrt(rho2), rho2, 1.0 + evm1)
-            if br == 3 or not policy:
-                B = lib.orc_magfield(geom.ref, rho2)
+            b = C.c_int(-1)
+            v = lib.orc_igrand_B(geom.ref, fermion, t, B, rho, 1.0 + evm1, C.byref(b))
+            if b.value == 3 or not policy:
                 TB = 0.30282212 * t * B
                 v = np.exp(-t) / t**3 * (evm1 - TB * TB / 3.0 if fermion else evm1 + TB * TB / 6.0)
             f[k] = v

@@ -568,6 +568,43 @@ double orc_smallT(const orc_geom *g, double rho, double rho2) {   /* Integrand.c
   return l2;
 }
 
+/* Field at x_cm (eB, Integrand.c:98-140 per tube).  Lattice geometries (an
+ * extension: the reference has isolated tubes only) sum the tubes of the
+ * image window that the Wilson loop sees, (2M+1)^2 tubes centred on the
+ * nearest one (window centre floor(c/l + 1/2)); *rho_near is the distance to
+ * that nearest tube (the radius the branch policy uses). */
+double orc_field_at(const orc_geom *g, double cx, double cy, double *rho_near) {
+  if (!(g->lattice > 0.0)) {
+    *rho_near = sqrt(cx * cx + cy * cy);
+    return orc_magfield(g, cx * cx + cy * cy);
+  }
+  const double L = g->lattice, m0 = floor(cx / L + 0.5), n0 = floor(cy / L + 0.5);
+  double B = 0.0;
+  for (int ia = -g->images; ia <= g->images; ++ia)
+    for (int ib = -g->images; ib <= g->images; ++ib) {
+      const double dx = cx - (m0 + ia) * L, dy = cy - (n0 + ib) * L;
+      B += orc_magfield(g, dx * dx + dy * dy);
+    }
+  const double dx = cx - m0 * L, dy = cy - n0 * L;
+  *rho_near = sqrt(dx * dx + dy * dy);
+  return B;
+}
+
+/* Tfunc's integrand (Integrand.c:252-305) at a given field B and radius rho. */
+double orc_igrand_B(const orc_geom *g, int fermion, double T, double B, double rho, double EV, int *branch) {
+  double TB = ORC_E * T * B, sT = orc_smallT(g, rho, rho * rho);
+  if (T < 0.1 && T < 0.01 * sT) { *branch = 0; return fermion == 1 ? orc_smTferm(B, T) : orc_smTscal(B, T); }
+  if (T < 0.01 * sT) { *branch = 1; return TB > 1.0e-6 ? orc_exact(T, B, fermion) : 0.0; }
+  if (T > 49.0) {
+    *branch = 2;
+    if (fermion == 1) return -1.0 * exp(-ORC_M2 * T) / (3.0 * T) * ORC_E * ORC_E * B * B;
+    return exp(-ORC_M2 * T) / (6.0 * T) * ORC_E * ORC_E * B * B;
+  }
+  *branch = 3;
+  if (fermion == 1) return exp(-ORC_M2 * T) / (T * T * T) * (EV - 1.0 - 1.0 / 3.0 * TB * TB);
+  return exp(-ORC_M2 * T) / (T * T * T) * (EV - 1.0 + 1.0 / 6.0 * TB * TB);
+}
+
 double orc_igrand(const orc_geom *g, int fermion, double T, double rho, double rho2,
                   double EV, int *branch) {
   double B = orc_magfield(g, rho2), TB = ORC_E * T * B, sT = orc_smallT(g, rho, rho2);

@@ -90,6 +90,10 @@ double orc_magfield(const orc_geom *g, double rho2);
 double orc_smallT(const orc_geom *g, double rho, double rho2);
 /* One group's integrand at T given the worldline estimate EV (used only on
  * the worldline branch); returns the branch id through *branch. */
+/* eB at x_cm summed over a lattice geometry's image window (isolated: the
+ * tube's own field); *rho_near = distance to the nearest tube. */
+double orc_field_at(const orc_geom *g, double cx, double cy, double *rho_near);
+double orc_igrand_B(const orc_geom *g, int fermion, double T, double B, double rho, double EV, int *branch);
 double orc_igrand(const orc_geom *g, int fermion, double T, double rho, double rho2,
                   double EV, int *branch);
 

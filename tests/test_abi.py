@@ -126,3 +126,20 @@ def test_leff_groups_matches_oracle(wfl, orc):
                 b = orc.leff_groups(orc.Geom(orc.GAUSSIAN, F=1.0, l2=0.25), fermion, T, rho, 0.0, gs, counts,
                                     policy)
                 assert np.allclose(a, b, rtol=1e-12, atol=1e-300)
+
+
+def test_leff_groups_lattice_matches_oracle(wfl, orc):
+    """Lattice geometries: the field in the counterterm and the branch policy
+    is the image-window sum, in the host L_eff and the oracle alike."""
+    rng = np.random.default_rng(2)
+    T = np.logspace(-3, 2, 20)
+    gs = -np.abs(rng.normal(size=(20, 3, 3))) * 1e-2
+    counts = np.full(3, 128, np.int32)
+    for lat, M in ((1.5, 1), (0.4, 2)):
+        for cx, cy in ((0.1, 0.2), (0.7, 1.3), (-2.2, 0.45)):
+            for fermion in (0, 1):
+                a = wfl.leff_groups(wfl.Geometry(kind=wfl.GAUSSIAN, F=1.0, l2=0.25, lattice=lat, images=M), fermion,
+                                    T, cx, cy, gs, counts, 1)
+                b = orc.leff_groups(orc.Geom(orc.GAUSSIAN, F=1.0, l2=0.25, lattice=lat, images=M), fermion, T, cx,
+                                    cy, gs, counts, 1)
+                assert np.allclose(a, b, rtol=1e-12, atol=1e-300)

@@ -231,3 +231,27 @@ def test_host_closed_forms_match_oracle(orc, wfl):
         assert wfl.lib.Exact(T, B, ferm) == orc.lib.orc_exact(T, B, ferm)
         assert wfl.lib.smTscal(B, T) == orc.lib.orc_smTscal(B, T)
         assert wfl.lib.smTferm(B, T) == orc.lib.orc_smTferm(B, T)
+
+
+def test_lattice_field_window(orc):
+    """The image-window field: one tube's field when the window is a single
+    tube or the lattice is wide; periodic in x_cm; and its cell average is
+    the flux per cell, eB = 2 F f'(rho^2) integrating to 2 pi F / e over the
+    plane (Integrand.c:137), when the window holds all the flux."""
+    iso = orc.Geom(orc.GAUSSIAN, F=1.0, l2=0.25)
+    one = orc.Geom(orc.GAUSSIAN, F=1.0, l2=0.25, lattice=1.0, images=0)
+    B1, r1 = orc.field_at(one, 0.3, -0.2)
+    Bi, ri = orc.field_at(iso, 0.3, -0.2)
+    assert B1 == pytest.approx(Bi, rel=1e-15) and r1 == pytest.approx(ri, rel=1e-15)
+    # x_cm beyond the first cell: the window follows it (nearest tube at (2, -1))
+    B2, r2 = orc.field_at(one, 2.3, -1.2)
+    assert B2 == pytest.approx(Bi, rel=1e-12) and r2 == pytest.approx(ri, rel=1e-12)
+    wide = orc.Geom(orc.GAUSSIAN, F=1.0, l2=0.25, lattice=50.0, images=2)
+    assert orc.field_at(wide, 0.3, -0.2)[0] == pytest.approx(Bi, rel=1e-12)
+    # cell average: lattice l = 2, lambda = 0.5, 5 x 5 window
+    lat = orc.Geom(orc.GAUSSIAN, F=1.0, l2=0.25, lattice=2.0, images=2)
+    n = 64
+    c = (np.arange(n) + 0.5) * (2.0 / n)
+    avg = np.mean([orc.field_at(lat, x, y)[0] for x in c for y in c])
+    e = 0.30282212
+    assert avg == pytest.approx(2 * np.pi * 1.0 / e / 4.0, rel=1e-6)

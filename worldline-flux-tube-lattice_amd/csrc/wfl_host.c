@@ -258,9 +258,31 @@ static double smallT_geom(const wfl_geometry *g, double rho) {
   return l2;
 }
 
-/* The integrand of one group at T given <W> - 1 (Integrand.c:252-305).
- * Returns the branch taken: 0 small-T series, 1 constant-field closed form,
- * 2 large-T asymptote, 3 worldline estimate. */
+/* Field at x_cm.  Isolated tube: its own field at rho^2 = |x_cm|^2.  A
+ * lattice geometry (an extension: the reference has isolated tubes only)
+ * sums the tubes of the image window the Wilson loop sees, (2M+1)^2 tubes
+ * centred on the nearest one (window centre floor(c/l + 1/2), as the
+ * kernels); *rho_near is the distance to that nearest tube. */
+static double field_at(const wfl_geometry *g, double cx, double cy, double *rho_near) {
+  if (!(g->lattice > 0.0)) {
+    *rho_near = sqrt(cx * cx + cy * cy);
+    return magfield_geom(g, cx * cx + cy * cy);
+  }
+  const double L = g->lattice, m0 = floor(cx / L + 0.5), n0 = floor(cy / L + 0.5);
+  double B = 0.0;
+  for (int ia = -g->images; ia <= g->images; ++ia)
+    for (int ib = -g->images; ib <= g->images; ++ib) {
+      const double dx = cx - (m0 + ia) * L, dy = cy - (n0 + ib) * L;
+      B += magfield_geom(g, dx * dx + dy * dy);
+    }
+  const double dx = cx - m0 * L, dy = cy - n0 * L;
+  *rho_near = sqrt(dx * dx + dy * dy);
+  return B;
+}
+
+/* The integrand of one group at T given <W> - 1 (Integrand.c:252-305) for
+ * an isolated tube at rho^2.  Returns the branch taken: 0 small-T series,
+ * 1 constant-field closed form, 2 large-T asymptote, 3 worldline estimate. */
 static double igrand_geom(const wfl_geometry *g, int fermion, double T, double rho, double rho2,
                           double evm1, int *branch) {
   double B = magfield_geom(g, rho2), TB = WFL_E * T * B, sT = smallT_geom(g, rho);
@@ -384,10 +406,10 @@ int wfl_leff_groups(const wfl_geometry *g, int fermion, int nT, const double *T,
     if (rc) { free(f); return rc; }
     gq.l2 = e[0];
     gq.F = e[1];
+    gq.lattice = e[5];
     Te[k] = e[4] * e[4];
-    rr[k] = e[2] * e[2] + e[3] * e[3];
-    B[k] = magfield_geom(&gq, rr[k]);
-    sT[k] = smallT_geom(&gq, sqrt(rr[k]));
+    B[k] = field_at(&gq, e[2], e[3], &rr[k]);
+    sT[k] = smallT_geom(&gq, rr[k]);
   }
   for (int q = 0; q < ngroups; ++q) {
     for (int k = 0; k < nT; ++k) {

@@ -214,3 +214,23 @@ def test_gaussian_mixed_small_loops_in_core(gpu, orc, loops, spinor):
         if spinor:
             assert relerr(pl[t, :, 3:], rpl[t, :, 3:]) < 1e-6
     print(f"  max phase err {max(relerr(pl[t, :, :3], rpl[t, :, :3]) for t in range(pl.shape[0])):.2e}")
+
+
+def test_gaussian_mixed_small_loops_lattice(gpu, orc, loops):
+    """The same in a lattice (spacing 1.5 lambda, 3 x 3 tubes): the kernel
+    picks the node per image tube and wavefront.  Bar 2e-5: a small loop
+    (scaled size s') at scaled distance |c'| from a neighbouring tube sums
+    that tube's gauge potential over terms ~|c'|/s' times its flux through
+    the loop, so the FP32 rounding of the coordinates (not the node) gives
+    a relative error ~1e-7 |c'|/s' in that tube's share of the phase:
+    |c'|/s' ~ 100 here at T = 1e-3."""
+    geoms = [gpu.Geometry(kind=gpu.GAUSSIAN, F=1.0, l2=4.0, lattice=3.0, images=1)]
+    og = [orc.Geom(orc.GAUSSIAN, F=1.0, l2=4.0, lattice=3.0, images=1)]
+    tg, tcx, tcy, tT = tuples([0.02, 0.4, 1.5, 2.9], [1e-3, 1e-2, 0.3])
+    tcy = 0.5 * tcx
+    gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, 1, 2, 1, gpu.MIXED, 1, (tg, tcx, tcy, tT))
+    for t in range(pl.shape[0]):
+        assert relerr(pl[t, :, :3], rpl[t, :, :3]) < 2e-5, t
+        assert relerr(pl[t, :, 3:], rpl[t, :, 3:]) < 2e-6, t
+    print(f"  max phase err {max(relerr(pl[t, :, :3], rpl[t, :, :3]) for t in range(pl.shape[0])):.2e}")
+    print(f"  max spin err {max(relerr(pl[t, :, 3:], rpl[t, :, 3:]) for t in range(pl.shape[0])):.2e}")

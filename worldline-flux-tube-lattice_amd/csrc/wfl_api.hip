@@ -494,10 +494,12 @@ static void fill_tuple(double sg, const DevGeom& dg, int gi, double cx, double c
   }
 }
 
-// Accurate FP32 Gaussian nodes for a tuple (scaled units, sigma != 1): the
-// loops are small (s' = sigma sqrt(T) < 1/2) and some point can come within
-// |z'| < 0.75 (u' < 0.56) of the nearest tube axis, given the ensemble's
-// largest radius rmax.  The plain quotient's phase error for a loop in the
+// Tuples whose loops can reach a tube core with the accurate FP32 Gaussian
+// node (scaled units, sigma != 1): the loops are small (s' = sigma sqrt(T)
+// < 1/2) and some point can come within |z'| < 0.75 (u' < 0.56) of the
+// nearest tube axis, given the ensemble's largest radius rmax.  The plan
+// puts such tuples in lane groups of their own; the kernels decide per image
+// tube and wavefront (the same test on each lane).  The plain quotient's phase error for a loop in the
 // core falls roughly as 1/s'^2: ~4e-7 relative in W - 1 at s' = 0.19, ~1e-8
 // at s' = 0.6 (5120 loops x 4096 points, lambda = 2); from s' = 1/2 up it is
 // below 1e-7, and loops away from the core have no nodes there.
@@ -511,10 +513,7 @@ static int tuple_needs_accurate(const DevGeom& dg, const DevTuple& d, double sg,
     dx -= L * floor(dx / L + 0.5);
     dy -= L * floor(dy / L + 0.5);
   }
-  if (!(sqrt(dx * dx + dy * dy) - s * (double)rmax < 0.75)) return 0;
-  // isolated tube: every point is within 0.75 + 2 s' rmax < 10 of the axis,
-  // so u' < 100 and the bounded accurate node applies; else the clamped one
-  return (dg.lattice <= 0.0 && 0.75 + 2.0 * s * (double)rmax < 9.5) ? 1 : 2;
+  return (sqrt(dx * dx + dy * dy) - s * (double)rmax < 0.75) ? 1 : 0;
 }
 
 extern "C" int wfl_effective_tuple(const wfl_geometry* g, double cx, double cy, double T, double out[6]) {
@@ -705,6 +704,7 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
   for (int g = 0; g < b->ngeom; ++g) {
     const wfl_geometry& G = b->geoms[g];
     fill_geom(G, &sblob, true, geoms[g]);
+    if (b->precision == WFL_MIXED && G.kind == WFL_GAUSSIAN) { geoms[g].acc_on = 1; geoms[g].acc_rmax = rmax; }
     if (G.kind != WFL_GAUSSIAN) p->prof_gauss = 0;
     if (G.kind != WFL_SPLINE) all_spline = false;
   }
@@ -1137,10 +1137,12 @@ extern "C" int wfl_internal_ev_launch(const void* dev_aos, int Nl, int N, const 
   fill_geom(gg, g->kind == WFL_SPLINE ? &sb : nullptr, scaled, dg);
   DevTuple dt;
   fill_tuple(unit_scale(gg, scaled), dg, 0, (double)cx, (double)cy, (double)rtT, Nl, dt);
-  if (scaled) {
+  if (scaled && g->kind == WFL_GAUSSIAN) {
     float rmax = 0.0f;
     rc = loops_rmax(g_ev.L, &rmax);
     if (rc) return rc;
+    dg.acc_on = 1;
+    dg.acc_rmax = rmax;
     dt.acc = tuple_needs_accurate(dg, dt, unit_scale(gg, scaled), rmax);
   }
   HIP_TRY(hipMemcpy(g_ev.d_geom, &dg, sizeof dg, hipMemcpyHostToDevice));

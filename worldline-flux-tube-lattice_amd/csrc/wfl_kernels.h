@@ -225,14 +225,57 @@ __device__ __forceinline__ const float4* stage_splines(const float4* __restrict_
 }
 
 // ------------------------------------------------------ lane per tuple
-// Class sums of one loop over all image tubes for one lane's tuple: the
+// Class sums of one loop around one image tube for one lane's tuple: the
 // points in 32-point blocks (FP32 partial sums, added to the FP64 P and S
 // after each block), four points per step (wave-uniform records).  ACC
 // selects the FP32 Gaussian node (0 plain, 1 and 2 accurate).
 template <class R, int PROF, int SPINOR, int MODE, int NSUB, int ACC>
+__device__ __forceinline__ void image_sums(const R* __restrict__ rl, int N, R ccx, R ccy, const LaneC<R>& lc,
+                                           int kind, const GP<R>& gp, const SplineRef& sp, int nsub_rt,
+                                           double P[3], double S[3]) {
+  for (int i0 = 0; i0 < N; i0 += 32) {
+    R p[3] = {(R)0, (R)0, (R)0}, q[3] = {(R)0, (R)0, (R)0};
+    const int i1 = i0 + 32 < N ? i0 + 32 : N;
+    for (int i = i0; i < i1; i += 4) {
+      const R* r = rl + (size_t)i * REC;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int c = (u & 1) ? 0 : (u == 0 ? 2 : 1);
+        point_eval<R, PROF, SPINOR, MODE, NSUB, ACC>(r[REC * u + 0], r[REC * u + 1], r[REC * u + 2],
+                                                     r[REC * u + 3], r[REC * u + 4], r[REC * u + 5], ccx, ccy,
+                                                     lc, kind, gp, sp, nsub_rt, p[c], q[c]);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { P[c] += (double)p[c]; if (SPINOR) S[c] += (double)q[c]; }
+  }
+}
+
+// The FP32 Gaussian node for image tube c (centre-relative offset c' in
+// kernel units), one choice for the whole wavefront (ballots), so every lane
+// runs the same code: the accurate node when some lane's loops are small
+// (s' < 1/2) and can reach that tube's core (|c'| - s' rmax < 0.75, i.e.
+// u' < 0.56), the bounded form (1) when every lane's points stay within
+// u' < 90 of the tube (|c'| + s' rmax < 9.5), else the clamped form (2).
+template <class R>
+__device__ __forceinline__ int image_acc(const DevGeom& gm, const LaneC<R>& lc, R ccx, R ccy) {
+  if constexpr (sizeof(R) != 4) {
+    return 0;
+  } else {
+    if (!gm.acc_on) return 0;
+    const float d = __builtin_sqrtf(fmaf(ccx, ccx, ccy * ccy));
+    const float reach = lc.s * gm.acc_rmax;
+    if (__builtin_amdgcn_ballot_w64(lc.s < 0.5f && d - reach < 0.75f) == 0) return 0;
+    return __builtin_amdgcn_ballot_w64(!(d + reach < 9.5f)) == 0 ? 1 : 2;
+  }
+}
+
+// Class sums of one loop over all image tubes for one lane's tuple.
+template <class R, int PROF, int SPINOR, int MODE, int NSUB>
 __device__ __forceinline__ void loop_sums(const R* __restrict__ rl, int N, int M, bool lattice, const DevTuple& tp,
-                                          R cx, R cy, R lat, const LaneC<R>& lc, int kind, const GP<R>& gp,
-                                          const SplineRef& sp, int nsub_rt, double P[3], double S[3]) {
+                                          const DevGeom& gm, R cx, R cy, R lat, const LaneC<R>& lc, int kind,
+                                          const GP<R>& gp, const SplineRef& sp, int nsub_rt, double P[3],
+                                          double S[3]) {
   for (int ia = -M; ia <= M; ++ia) {
     for (int ib = -M; ib <= M; ++ib) {
       R ccx = cx, ccy = cy;
@@ -240,22 +283,13 @@ __device__ __forceinline__ void loop_sums(const R* __restrict__ rl, int N, int M
         ccx = fma(-(R)(tp.m0 + ia), lat, cx);
         ccy = fma(-(R)(tp.n0 + ib), lat, cy);
       }
-      for (int i0 = 0; i0 < N; i0 += 32) {
-        R p[3] = {(R)0, (R)0, (R)0}, q[3] = {(R)0, (R)0, (R)0};
-        const int i1 = i0 + 32 < N ? i0 + 32 : N;
-        for (int i = i0; i < i1; i += 4) {
-          const R* r = rl + (size_t)i * REC;
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int c = (u & 1) ? 0 : (u == 0 ? 2 : 1);
-            point_eval<R, PROF, SPINOR, MODE, NSUB, ACC>(r[REC * u + 0], r[REC * u + 1], r[REC * u + 2],
-                                                         r[REC * u + 3], r[REC * u + 4], r[REC * u + 5], ccx, ccy,
-                                                         lc, kind, gp, sp, nsub_rt, p[c], q[c]);
-          }
-        }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) { P[c] += (double)p[c]; if (SPINOR) S[c] += (double)q[c]; }
-      }
+      const int acc = (PROF == P_GAUSSIAN) ? image_acc<R>(gm, lc, ccx, ccy) : 0;
+      if (acc == 1)
+        image_sums<R, PROF, SPINOR, MODE, NSUB, 1>(rl, N, ccx, ccy, lc, kind, gp, sp, nsub_rt, P, S);
+      else if (acc == 2)
+        image_sums<R, PROF, SPINOR, MODE, NSUB, 2>(rl, N, ccx, ccy, lc, kind, gp, sp, nsub_rt, P, S);
+      else
+        image_sums<R, PROF, SPINOR, MODE, NSUB, 0>(rl, N, ccx, ccy, lc, kind, gp, sp, nsub_rt, P, S);
     }
   }
 }
@@ -294,20 +328,13 @@ __device__ __forceinline__ void tuple_task(
   const R lat = sizeof(R) == 4 ? (R)gm.latticef : (R)gm.lattice;
   const R cx = sizeof(R) == 4 ? (R)tp.cx : (R)tp.cxd, cy = sizeof(R) == 4 ? (R)tp.cy : (R)tp.cyd;
 
-  // accurate Gaussian nodes: uniform over the lane group (the host groups them)
-  const int acc = (sizeof(R) == 4 && PROF == P_GAUSSIAN) ? __builtin_amdgcn_readfirstlane(tp.acc) : 0;
   const bool lattice = gm.lattice > 0.0;
 
   double G[3] = {0.0, 0.0, 0.0};
   for (int l = loop0; l < loop0 + nloop; ++l) {
     const R* __restrict__ rl = (const R*)__builtin_assume_aligned(rec + (size_t)l * N * REC, REC * sizeof(R));
     double P[3] = {0.0, 0.0, 0.0}, S[3] = {0.0, 0.0, 0.0};
-    if (acc == 1)
-      loop_sums<R, PROF, SPINOR, MODE, NSUB, 1>(rl, N, M, lattice, tp, cx, cy, lat, lc, kind, gp, sp, nsub_rt, P, S);
-    else if (acc == 2)
-      loop_sums<R, PROF, SPINOR, MODE, NSUB, 2>(rl, N, M, lattice, tp, cx, cy, lat, lc, kind, gp, sp, nsub_rt, P, S);
-    else
-      loop_sums<R, PROF, SPINOR, MODE, NSUB, 0>(rl, N, M, lattice, tp, cx, cy, lat, lc, kind, gp, sp, nsub_rt, P, S);
+    loop_sums<R, PROF, SPINOR, MODE, NSUB>(rl, N, M, lattice, tp, gm, cx, cy, lat, lc, kind, gp, sp, nsub_rt, P, S);
     double sc[3], fe[3];
     finalize_loop(P, S, gm.F, s_phase, tp.sd * tp.sd, spin_scale, n, N, SPINOR, sc, fe);
     if (!SPLIT) {
@@ -369,32 +396,23 @@ __global__ __launch_bounds__(256) void k_tuple(WFL_K_TUPLE_PARAMS) {
 }
 
 // ------------------------------------------------------- wave per loop
-// One loop's sums for one tuple, lanes striding over points (lane l takes
-// points l + 64k, all of class l & 3), in blocks of 8 points per lane.
+// One loop's sums for one tuple around one image tube, lanes striding over
+// points (lane l takes points l + 64k, all of class l & 3), in blocks of 8
+// points per lane.
 template <class R, int PROF, int SPINOR, int MODE, int NSUB, int ACC>
-__device__ __forceinline__ void wave_loop_sums(const R* __restrict__ rl, int N, int M, bool lattice,
-                                               const DevTuple& tp, R cx, R cy, R lat, const LaneC<R>& lc, int kind,
-                                               const GP<R>& gp, const SplineRef& sp, int nsub_rt, int lane,
-                                               double& P, double& S) {
-  for (int ia = -M; ia <= M; ++ia) {
-    for (int ib = -M; ib <= M; ++ib) {
-      R ccx = cx, ccy = cy;
-      if (lattice) {
-        ccx = fma(-(R)(tp.m0 + ia), lat, cx);
-        ccy = fma(-(R)(tp.n0 + ib), lat, cy);
-      }
-      for (int i0 = lane; i0 < N; i0 += 64 * 8) {
-        R p = (R)0, q = (R)0;
-        const int i1 = i0 + 64 * 8 < N ? i0 + 64 * 8 : N;
-        for (int i = i0; i < i1; i += 64) {
-          const R* r = rl + (size_t)i * REC;
-          point_eval<R, PROF, SPINOR, MODE, NSUB, ACC>(r[0], r[1], r[2], r[3], r[4], r[5], ccx, ccy, lc, kind, gp,
-                                                       sp, nsub_rt, p, q);
-        }
-        P += (double)p;
-        S += (double)q;
-      }
+__device__ __forceinline__ void wave_image_sums(const R* __restrict__ rl, int N, R ccx, R ccy, const LaneC<R>& lc,
+                                                int kind, const GP<R>& gp, const SplineRef& sp, int nsub_rt,
+                                                int lane, double& P, double& S) {
+  for (int i0 = lane; i0 < N; i0 += 64 * 8) {
+    R p = (R)0, q = (R)0;
+    const int i1 = i0 + 64 * 8 < N ? i0 + 64 * 8 : N;
+    for (int i = i0; i < i1; i += 64) {
+      const R* r = rl + (size_t)i * REC;
+      point_eval<R, PROF, SPINOR, MODE, NSUB, ACC>(r[0], r[1], r[2], r[3], r[4], r[5], ccx, ccy, lc, kind, gp, sp,
+                                                   nsub_rt, p, q);
     }
+    P += (double)p;
+    S += (double)q;
   }
 }
 
@@ -433,16 +451,22 @@ __global__ __launch_bounds__(256) void k_loop(
   const R* __restrict__ rl = rec + (size_t)l * N * REC;
 
   double P = 0.0, S = 0.0;      // this lane's class: lane & 3 (points i = lane + 64k)
-  const int acc = (sizeof(R) == 4 && PROF == P_GAUSSIAN) ? tp.acc : 0;
-  if (acc == 1)
-    wave_loop_sums<R, PROF, SPINOR, MODE, NSUB, 1>(rl, N, M, gm.lattice > 0.0, tp, cx, cy, lat, lc, kind, gp, sp,
-                                                   nsub_rt, lane, P, S);
-  else if (acc == 2)
-    wave_loop_sums<R, PROF, SPINOR, MODE, NSUB, 2>(rl, N, M, gm.lattice > 0.0, tp, cx, cy, lat, lc, kind, gp, sp,
-                                                   nsub_rt, lane, P, S);
-  else
-    wave_loop_sums<R, PROF, SPINOR, MODE, NSUB, 0>(rl, N, M, gm.lattice > 0.0, tp, cx, cy, lat, lc, kind, gp, sp,
-                                                   nsub_rt, lane, P, S);
+  for (int ia = -M; ia <= M; ++ia) {
+    for (int ib = -M; ib <= M; ++ib) {
+      R ccx = cx, ccy = cy;
+      if (gm.lattice > 0.0) {
+        ccx = fma(-(R)(tp.m0 + ia), lat, cx);
+        ccy = fma(-(R)(tp.n0 + ib), lat, cy);
+      }
+      const int acc = (PROF == P_GAUSSIAN) ? image_acc<R>(gm, lc, ccx, ccy) : 0;
+      if (acc == 1)
+        wave_image_sums<R, PROF, SPINOR, MODE, NSUB, 1>(rl, N, ccx, ccy, lc, kind, gp, sp, nsub_rt, lane, P, S);
+      else if (acc == 2)
+        wave_image_sums<R, PROF, SPINOR, MODE, NSUB, 2>(rl, N, ccx, ccy, lc, kind, gp, sp, nsub_rt, lane, P, S);
+      else
+        wave_image_sums<R, PROF, SPINOR, MODE, NSUB, 0>(rl, N, ccx, ccy, lc, kind, gp, sp, nsub_rt, lane, P, S);
+    }
+  }
   // lanes with equal (lane & 3) hold the same class: reduce over them
 #pragma unroll
   for (int m = 4; m < 64; m <<= 1) {

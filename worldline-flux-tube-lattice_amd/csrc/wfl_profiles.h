@@ -37,6 +37,8 @@ struct DevGeom {
   float Ff, l2f, inv_l2f, af, lminf, q1f, q2f;      // FP32 generic-profile values
   float latticef;                                   // lattice spacing in FP32 kernel units
   float c8f, c10f;                                  // reference clamps 1e-8, 1e-10 in FP32 kernel units
+  int acc_on;                                       // FP32 Gaussian: accurate node where loops reach a core
+  float acc_rmax;                                   // largest loop radius of the ensemble (unit loops)
 };
 
 // One evaluation tuple; the image-window centre (m0, n0) is computed on the
@@ -44,7 +46,7 @@ struct DevGeom {
 struct DevTuple {
   int geom, m0, n0, nloops;
   float cx, cy, s;              // FP32 kernel units
-  int acc;                      // FP32 Gaussian node: 0 plain, 1 accurate (u' < 100), 2 accurate (any u')
+  int acc;                      // 1: the loops can reach a tube core with the accurate node (groups lanes)
   double cxd, cyd, sd, padd;    // effective values (FP64 kernels)
 };
 
@@ -97,8 +99,8 @@ template <> struct GP<double> {
 // every u' (the device of Kahan's expm1 from exp and log).  u' is held at or
 // above 1e-6 in the exponential (g' there is ln 2 to 3.5e-7; e for such a
 // node, within 1e-3 of the axis, is off by < 7e-7).  ACC = 1 needs u' < 100
-// (e stays a normal number), which the host guarantees for the tuples it
-// routes there; ACC = 2 also clamps u' at 100 in the exponential and adds
+// (e stays a normal number), which the caller establishes for the whole
+// wavefront; ACC = 2 also clamps u' at 100 in the exponential and adds
 // u' - 100 back to the denominator (g' = 1/u' there to 2^-100).
 template <int ACC>
 __device__ __forceinline__ float gauss_g(float u, double, float& e) {

@@ -81,6 +81,16 @@ double *Tfunc(double T, void *p, int *order);
 double *integrateT(int n, void *p);
 /* getwl.c:6 -- text loops into the reference's float4 AoS layout */
 int getwl(float4 *worldlines, char *filename, int Nl, int Nppl, int Dim);
+/* EffAct.c:15-306 -- the rho-integral drivers (also in libwfl.so) */
+double *rhogrand(double rho, void *p);
+void testvalrho(double *val, int n, double t);
+double *transformRho(double rhot, void *p);
+double *integrateRhotoa(int n, void *p);
+double *integrateRho(int n, void *p);
+double PeriodicCA(double l2);
+double FixFluxCA(double l2);
+double ClassAction(double F, double l2);
+double *EAction(int Npoints, void *p);
 /* Integrand.c:20-172 closed forms */
 double Exact(double T, double B, int fermion);
 double MagField(double rho2, void *p);
@@ -101,6 +111,7 @@ static inline int wfl_sync_intT(const struct Wparams *p) {
   g.lattice = 0.0;
   g.images = 0;
   g.spline_table = (profile == spline) ? 0 : -1;
+  wfl_set_verbosity(verbosity);
   return wfl_set_profile(&g, Nspline);
 }
 

@@ -95,7 +95,10 @@ typedef struct {
   const double *tup_T;  /* [ntup] proper time T (quantized: sqrt(T) to FP32) */
   const int *geom_nloops; /* [ngeom] loops used per geometry (NULL: all Nl) */
   int want_perloop;     /* also keep per-loop (scal[3], ferm[3]) FP64 outputs */
-  int mapping;          /* 0: autotune; 1: lane per tuple; 2: wave per loop;
+  int mapping;          /* 0: chosen at creation from the batch's size (small
+                           batches split loop groups into several tasks;
+                           wfl_plan_autotune measures the alternatives);
+                           1: lane per tuple; 2: wave per loop;
                            3: lane per tuple with W-1 in a second kernel */
 } wfl_batch;
 
@@ -202,6 +205,20 @@ double wfl_effective_T(const wfl_geometry *g, double T);
  * /(4 pi) spinor. */
 int wfl_leff_groups(const wfl_geometry *g, int fermion, int nT, const double *T, double cx, double cy,
                     const double *gsum, const int *counts, int ngroups, int policy, double *leff);
+
+/* Effective action of an isolated analytic tube with the batched engine:
+ * EAction's rho quadrature (EffAct.c:265-306, integrateRho with Npoints
+ * intervals, or integrateRhotoa with 2 Npoints for periodic/fixflux) over
+ * integrateT's nT-interval rule in t (Tint.c:100-175), Tfunc's branch policy
+ * at every node; the worldline nodes of the whole (rho, T) grid run as one
+ * plan.  groups_out [ceil(Nl/group)] (may be NULL) gets each group's action,
+ * *result their mean and *se its standard error (the reference's EAvsLambda:
+ * and ActRatio: lines are printed), *classical ClassAction (EffAct.c:231-263).
+ * Not for spline or lattice geometries (WFL_ERR_ARG). */
+int wfl_eaction(const wfl_geometry *g, wfl_loops *L, int fermion, int mode, int precision, int nsub, int group,
+                int Npoints, int nT, double *groups_out, double *result, double *se, double *classical);
+/* Output level of the rho-integral drivers (EffAct.c's verbosity; default 1). */
+int wfl_set_verbosity(int v);
 
 #ifdef __cplusplus
 }

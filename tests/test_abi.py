@@ -143,3 +143,20 @@ def test_leff_groups_lattice_matches_oracle(wfl, orc):
                 b = orc.leff_groups(orc.Geom(orc.GAUSSIAN, F=1.0, l2=0.25, lattice=lat, images=M), fermion, T, cx,
                                     cy, gs, counts, 1)
                 assert np.allclose(a, b, rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("kind,F,l2", [("STEP", 1.0, 1.0), ("SMOOTH", 0.5, 2.0), ("QUADRATIC", 2.0, 0.5),
+                                       ("GAUSSIAN", 1.0, 0.25)])
+def test_class_action_is_field_energy(wfl, orc, kind, F, l2):
+    """ClassAction (EffAct.c:231-263) is minus half the field energy,
+    -1/2 int B^2 d^2x, with eB = 2 F f'(rho^2) (Integrand.c:137): pinned by
+    quadrature of the oracle's field."""
+    from scipy.integrate import quad
+    wfl.set_profile(wfl.Geometry(kind=getattr(wfl, kind), F=F, l2=l2))
+    g = orc.Geom(getattr(orc, kind), F=F, l2=l2, quantize=False)
+    lam = np.sqrt(l2)
+    pts = [lam] if kind in ("STEP", "QUADRATIC") else None
+    val, _ = quad(lambda r: orc.lib.orc_magfield(g.ref, r * r) ** 2 * 2 * np.pi * r, 0.0, 40.0 * lam, points=pts,
+                  limit=400)
+    assert wfl.lib.ClassAction(F, l2) == pytest.approx(-0.5 * val, rel=1e-8)
+    wfl.set_profile(wfl.Geometry(kind=wfl.GAUSSIAN))

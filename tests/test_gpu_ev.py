@@ -147,3 +147,29 @@ def test_integrateT_matches_oracle(setup, orc):
         b = tfunc((1 - th) / th) / th / th
         want += h / 6.0 * (2 * a + 4 * b)
     assert np.allclose(got, want, rtol=1e-6, atol=1e-12)
+
+
+def test_batched_eaction_matches_drop_in(setup, orc):
+    """The effective action two ways on the same loops: EAction driving EV()
+    node by node (EffAct.c:265-306 -> integrateRho -> integrateT -> Tfunc ->
+    EV), and wfl_eaction with all (rho, T) worldline nodes in one plan.  The
+    drop-in narrows F and l2 to float where the batch uses its own FP32
+    parameter grid, a difference of ~1e-7 in the inputs."""
+    wfl, hip, x, y, d_wl, d_s, d_f, hs, hf = setup
+    F, l2, npts = 1.0, 1.0, 2
+    wfl.set_profile(wfl.Geometry(kind=wfl.GAUSSIAN, F=F, l2=l2))
+    wfl.set_numerics(wfl.MODE_REFERENCE, wfl.FP64, 50)
+    wfl.lib.wfl_ev_reset()
+    p = make_params(wfl, d_wl, d_s, d_f, hs, hf, F, l2, 0.0, 0, ng=2)
+    print("EAction drop-in", flush=True)
+    res = wfl.lib.EAction(npts, C.byref(p))
+    drop = np.ctypeslib.as_array(C.cast(res, C.POINTER(C.c_double)), shape=(2,)).copy()
+    L = wfl.Loops(x, y)
+    print("EAction batched", flush=True)
+    groups, mean, se, ca = wfl.eaction(wfl.Geometry(kind=wfl.GAUSSIAN, F=F, l2=l2), L, 0, wfl.MODE_REFERENCE,
+                                       wfl.FP64, 50, 128, npts, 40)
+    L.close()
+    assert ca == pytest.approx(wfl.lib.ClassAction(F, l2), rel=1e-12)
+    assert mean == pytest.approx(groups.mean(), rel=1e-12)
+    assert np.all(np.isfinite(groups))
+    assert np.allclose(groups, drop, rtol=1e-4, atol=0.0)

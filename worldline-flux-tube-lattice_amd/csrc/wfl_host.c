@@ -298,6 +298,16 @@ static double igrand_geom(const wfl_geometry *g, int fermion, double T, double r
   return exp(-WFL_M2 * T) / (T * T * T) * (evm1 + 1.0 / 6.0 * TB * TB);
 }
 
+/* The drop-in profile (wfl_set_profile) and Tfunc's integrand with its
+ * field, for the rho-integral drivers. */
+const wfl_geometry *wfl_internal_profile(void) { return &g_prof; }
+
+double wfl_internal_igrand(const wfl_geometry *g, int fermion, double T, double rho, double rho2, double evm1,
+                           int *branch, double *B) {
+  if (B) *B = magfield_geom(g, rho2);
+  return igrand_geom(g, fermion, T, rho, rho2, evm1, branch);
+}
+
 double *Tfunc(double T, void *p, int *order) {
   struct Wparams *params = (struct Wparams *)p;
   float rho2f = params->xcm.x * params->xcm.x + params->xcm.y * params->xcm.y;   /* float, Integrand.c:199 */

@@ -26,6 +26,13 @@ int wfl_internal_d2h(void *dst, const void *src, size_t bytes);
 /* Drop the loop copies cached for EV(). */
 void wfl_internal_ev_release(void);
 
+/* The drop-in profile set by wfl_set_profile / wfl_sync_intT. */
+const wfl_geometry *wfl_internal_profile(void);
+/* Tfunc's integrand for an isolated tube (Integrand.c:252-305), with the
+ * field it used in *B. */
+double wfl_internal_igrand(const wfl_geometry *g, int fermion, double T, double rho, double rho2, double evm1,
+                           int *branch, double *B);
+
 /* FP32 quantization used for every parameter the kernels see. */
 static inline double wfl_q32(double v) { return (double)(float)v; }
 

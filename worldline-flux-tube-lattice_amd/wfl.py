@@ -86,6 +86,8 @@ EXPORTS = [
     "wfl_effective_tuple",
     "wfl_leff_groups",
     "EV", "EVMean", "Tfunc", "integrateT", "getwl", "Exact", "MagField", "smTscal", "smTferm",
+    "rhogrand", "testvalrho", "transformRho", "integrateRhotoa", "integrateRho", "PeriodicCA", "FixFluxCA",
+    "ClassAction", "EAction", "wfl_eaction", "wfl_set_verbosity",
 ]
 
 
@@ -134,6 +136,16 @@ def _load():
         "EVMean": (None, [P, P, P, I, P, D, I]),
         "Tfunc": (P, [D, P, P]),
         "integrateT": (P, [I, P]),
+        "rhogrand": (P, [D, P]),
+        "transformRho": (P, [D, P]),
+        "integrateRhotoa": (P, [I, P]),
+        "integrateRho": (P, [I, P]),
+        "PeriodicCA": (D, [D]),
+        "FixFluxCA": (D, [D]),
+        "ClassAction": (D, [D, D]),
+        "EAction": (P, [I, P]),
+        "wfl_eaction": (I, [P, P, I, I, I, I, I, I, I, P, P, P, P]),
+        "wfl_set_verbosity": (I, [I]),
         "getwl": (I, [P, C.c_char_p, I, I, I]),
         "Exact": (D, [D, D, I]),
         "MagField": (D, [D, P]),
@@ -388,6 +400,20 @@ def leff_groups(geom: Geometry, fermion, T, cx, cy, gsum, counts, policy=1):
     check(lib.wfl_leff_groups(C.byref(g), int(fermion), len(T_eff), _ptr(T_eff), float(cx), float(cy),
                               _ptr(gsum), _ptr(counts), ng, int(policy), _ptr(out)))
     return out
+
+
+def eaction(geom: Geometry, loops: "Loops", fermion, mode=MODE_PHYSICAL, precision=MIXED, nsub=2, group=128,
+            npoints=20, nT=40):
+    """Effective action of an isolated tube with the batched engine
+    (EffAct.c's rho rule over Tint.c's T rule): (per-group actions, mean,
+    standard error, classical action)."""
+    ng = (loops.Nl + group - 1) // group
+    out = np.zeros(ng, np.float64)
+    r, se, ca = C.c_double(), C.c_double(), C.c_double()
+    g = geom.c()
+    check(lib.wfl_eaction(C.byref(g), loops.h, int(fermion), mode, precision, nsub, group, npoints, nT, _ptr(out),
+                          C.byref(r), C.byref(se), C.byref(ca)))
+    return out, r.value, se.value, ca.value
 
 
 def set_profile(geom: Geometry, nspline=512):

@@ -186,9 +186,11 @@ def accuracy(wfl, cfg, gs_mixed, gs_64):
     """Mixed against FP64 on the same loops.  Per geometry: the L_eff(x_cm)
     error relative to that geometry's largest |L_eff|, the relative error of
     the action (the sum over x_cm), the largest error in units of the
-    standard error of the mean over loop groups, and the median SEM/|L_eff|
-    (above ~0.1 the cell is dominated by Monte-Carlo noise: the sign problem
-    of cos(Phi) cosh(S) at strong fields)."""
+    standard error of the mean over loop groups among the x_cm where
+    |L_eff| >= 1e-3 of its largest value (far outside a tube both L_eff and
+    its SEM vanish), and the median SEM/|L_eff| (above ~0.1 the cell is
+    dominated by Monte-Carlo noise: the sign problem of cos(Phi) cosh(S) at
+    strong fields)."""
     res = {}
     for policy, tag in ((1, "tfunc_policy"), (0, "worldline_all_T")):
         Lm = leff_table(wfl, cfg, gs_mixed, policy)
@@ -199,10 +201,11 @@ def accuracy(wfl, cfg, gs_mixed, gs_64):
             ng = Ld[g].shape[-1]
             sem = Ld[g].std(-1, ddof=1) / np.sqrt(ng) if ng > 1 else None
             scale = max(np.max(np.abs(md)), 1e-300)
+            sig = np.abs(md) >= 1e-3 * scale
             per.append({"leff_max_err_over_max_leff": float(np.max(np.abs(mm - md)) / scale),
                         "action_rel_err": float(abs(mm.sum() - md.sum()) / max(abs(md.sum()), 1e-300)),
                         "max_err_over_sem": None if sem is None else
-                        float(np.max(np.abs(mm - md) / np.maximum(sem, 1e-300))),
+                        float(np.max((np.abs(mm - md) / np.maximum(sem, 1e-300))[sig])),
                         "median_sem_over_leff": None if sem is None else
                         float(np.median(sem / np.maximum(np.abs(md), 1e-300)))})
         res[tag] = per

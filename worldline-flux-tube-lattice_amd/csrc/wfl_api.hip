@@ -8,6 +8,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 #include "wfl_internal.h"
 #include "wfl_kernels.h"
@@ -898,7 +899,8 @@ struct TuneKey {
   bool operator==(const TuneKey& o) const { return memcmp(this, &o, sizeof o) == 0; }
 };
 struct TuneVal { int mapping, persistent, wpb, cpg; };
-static std::vector<std::pair<TuneKey, TuneVal>> g_tuned;
+static std::vector<std::pair<TuneKey, TuneVal>> g_tuned;   // guarded by g_tuned_mu
+static std::mutex g_tuned_mu;
 
 static int plan_apply(wfl_plan* p, int mapping, int persistent, int wpb, int cpg) {
   if (mapping == 3 && !p->d_perloop) {
@@ -930,14 +932,22 @@ extern "C" int wfl_plan_autotune(wfl_plan* p, void* stream, int* chosen) {
   key.spinor = p->b.spinor; key.mode = p->b.mode; key.nsub = p->b.nsub; key.prof_gauss = p->prof_gauss;
   key.lds_splines = p->lds_splines;
   key.ntasks1 = p->ntasks1;
-  for (auto& kv : g_tuned)
-    if (kv.first == key) {
-      int rc = plan_apply(p, kv.second.mapping, kv.second.persistent, kv.second.wpb, kv.second.cpg);
+  {
+    bool hit = false;
+    TuneVal v{};
+    {
+      std::lock_guard<std::mutex> lk(g_tuned_mu);
+      for (auto& kv : g_tuned)
+        if (kv.first == key) { v = kv.second; hit = true; break; }
+    }
+    if (hit) {
+      int rc = plan_apply(p, v.mapping, v.persistent, v.wpb, v.cpg);
       if (rc) return rc;
-      if (chosen) *chosen = tune_code(kv.second);
+      if (chosen) *chosen = tune_code(v);
       p->tuned = true;
       return WFL_OK;
     }
+  }
   // chunked groups only when one task per (lane group, loop group) gives
   // fewer than 4 wavefronts per SIMD
   std::vector<int> cpgs{1};
@@ -982,7 +992,10 @@ extern "C" int wfl_plan_autotune(wfl_plan* p, void* stream, int* chosen) {
   if (rc) return rc;
   rc = wfl_plan_run(p, stream);      // leave the plan's outputs from the chosen configuration
   if (rc) return rc;
-  g_tuned.push_back({key, best});
+  {
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
+    g_tuned.push_back({key, best});
+  }
   p->tuned = true;
   if (chosen) *chosen = tune_code(best);
   return WFL_OK;

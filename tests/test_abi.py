@@ -91,6 +91,32 @@ def test_gen_loops_and_text_reader_roundtrip(wfl, tmp_path):
     assert wfl.lib.wfl_read_text_loops(str(p).encode(), 4, 8, rx.ctypes.data, ry.ctypes.data) == -1
 
 
+def test_binary_loop_file_roundtrip(wfl, tmp_path):
+    """Binary loop file: header fields, exact round trip, a shape mismatch
+    and a corrupted byte are refused."""
+    Nl, N = 5, 16
+    x, y = wfl.gen_loops(7, Nl, N)
+    p = tmp_path / "wl.bin"
+    wfl.check(wfl.lib.wfl_loops_write(str(p).encode(), x.ctypes.data, y.ctypes.data, Nl, N, 7))
+    assert os.path.getsize(p) == 40 + 2 * 4 * Nl * N
+    hn, hN, hc = C.c_int(), C.c_int(), C.c_uint32()
+    wfl.check(wfl.lib.wfl_loops_read_header(str(p).encode(), C.addressof(hn), C.addressof(hN), C.addressof(hc)))
+    assert (hn.value, hN.value, hc.value) == (Nl, N, 7)
+    rx = np.zeros((Nl, N), np.float32)
+    ry = np.zeros((Nl, N), np.float32)
+    wfl.check(wfl.lib.wfl_loops_read(str(p).encode(), rx.ctypes.data, ry.ctypes.data, Nl, N))
+    assert np.array_equal(rx, x) and np.array_equal(ry, y)
+    assert wfl.lib.wfl_loops_read(str(p).encode(), rx.ctypes.data, ry.ctypes.data, Nl, N + 1) == -1
+    raw = bytearray(p.read_bytes())
+    raw[40 + 4 * N + 1] ^= 0x10
+    q = tmp_path / "bad.bin"
+    q.write_bytes(bytes(raw))
+    assert wfl.lib.wfl_loops_read(str(q).encode(), rx.ctypes.data, ry.ctypes.data, Nl, N) == -1
+    assert b"hash" in wfl.lib.wfl_last_error()
+    q.write_bytes(b"NOTALOOPFILE" + bytes(60))
+    assert wfl.lib.wfl_loops_read_header(str(q).encode(), None, None, None) == -1
+
+
 def test_effective_parameters_match_oracle(wfl, orc):
     """The engine's parameter narrowing (FP32 in the Gaussian fast path's
     scaled units, plain FP32 otherwise) restated independently in the
@@ -160,3 +186,10 @@ def test_class_action_is_field_energy(wfl, orc, kind, F, l2):
                   limit=400)
     assert wfl.lib.ClassAction(F, l2) == pytest.approx(-0.5 * val, rel=1e-8)
     wfl.set_profile(wfl.Geometry(kind=wfl.GAUSSIAN))
+
+
+def test_binary_loop_file_python_helpers(wfl, tmp_path):
+    x, y = wfl.gen_loops(3, 4, 32)
+    wfl.write_loops(tmp_path / "a.bin", x, y, config_id=3)
+    rx, ry, cid = wfl.read_loops(tmp_path / "a.bin")
+    assert cid == 3 and np.array_equal(rx, x) and np.array_equal(ry, y)

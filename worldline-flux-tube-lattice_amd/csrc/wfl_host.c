@@ -19,6 +19,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <stdint.h>
 #include "../../include/intT.h"
 #include "wfl_internal.h"
 
@@ -523,6 +524,84 @@ int wfl_read_text_loops(const char *path, int Nl, int N, float *x, float *y) {
       if (fscanf(fp, "%f ", &y[(size_t)l * N + j]) != 1) { fclose(fp); wfl_set_error("%s: short read at loop %d (y)", path, l); return WFL_ERR_ARG; }
   }
   fclose(fp);
+  return WFL_OK;
+}
+
+/* Binary loop file: a 40-byte little-endian header, then x[Nl*N] and
+ * y[Nl*N] as float32 (the engine's SoA layout).
+ *   char magic[8] = "WFLLOOP1"; uint32 version = 1, Nl, N, config_id;
+ *   uint64 hash = FNV-1a 64 over the x bytes then the y bytes; uint64 0. */
+static const char WFL_LOOP_MAGIC[8] = {'W', 'F', 'L', 'L', 'O', 'O', 'P', '1'};
+
+static uint64_t fnv1a64(uint64_t h, const void *data, size_t n) {
+  const unsigned char *p = (const unsigned char *)data;
+  for (size_t i = 0; i < n; ++i) { h ^= p[i]; h *= 1099511628211ULL; }
+  return h;
+}
+
+static uint64_t loops_hash(const float *x, const float *y, size_t n) {
+  uint64_t h = 14695981039346656037ULL;
+  h = fnv1a64(h, x, n * sizeof(float));
+  return fnv1a64(h, y, n * sizeof(float));
+}
+
+static void put_u32(unsigned char *b, uint32_t v) { for (int i = 0; i < 4; ++i) b[i] = (unsigned char)(v >> (8 * i)); }
+static void put_u64(unsigned char *b, uint64_t v) { for (int i = 0; i < 8; ++i) b[i] = (unsigned char)(v >> (8 * i)); }
+static uint32_t get_u32(const unsigned char *b) { uint32_t v = 0; for (int i = 3; i >= 0; --i) v = (v << 8) | b[i]; return v; }
+static uint64_t get_u64(const unsigned char *b) { uint64_t v = 0; for (int i = 7; i >= 0; --i) v = (v << 8) | b[i]; return v; }
+
+int wfl_loops_write(const char *path, const float *x, const float *y, int Nl, int N, uint32_t config_id) {
+  if (!path || !x || !y || Nl < 1 || N < 1) { wfl_set_error("wfl_loops_write: bad arguments"); return WFL_ERR_ARG; }
+  const size_t n = (size_t)Nl * N;
+  unsigned char h[40];
+  memcpy(h, WFL_LOOP_MAGIC, 8);
+  put_u32(h + 8, 1);
+  put_u32(h + 12, (uint32_t)Nl);
+  put_u32(h + 16, (uint32_t)N);
+  put_u32(h + 20, config_id);
+  put_u64(h + 24, loops_hash(x, y, n));
+  put_u64(h + 32, 0);
+  FILE *fp = fopen(path, "wb");
+  if (!fp) { wfl_set_error("cannot create %s", path); return WFL_ERR_ARG; }
+  int ok = fwrite(h, 1, sizeof h, fp) == sizeof h && fwrite(x, sizeof(float), n, fp) == n &&
+           fwrite(y, sizeof(float), n, fp) == n;
+  ok = (fclose(fp) == 0) && ok;
+  if (!ok) { wfl_set_error("%s: write failed", path); return WFL_ERR_ARG; }
+  return WFL_OK;
+}
+
+int wfl_loops_read_header(const char *path, int *Nl, int *N, uint32_t *config_id) {
+  if (!path) { wfl_set_error("wfl_loops_read_header: NULL path"); return WFL_ERR_ARG; }
+  FILE *fp = fopen(path, "rb");
+  if (!fp) { wfl_set_error("cannot open %s", path); return WFL_ERR_ARG; }
+  unsigned char h[40];
+  const size_t got = fread(h, 1, sizeof h, fp);
+  fclose(fp);
+  if (got != sizeof h || memcmp(h, WFL_LOOP_MAGIC, 8) != 0 || get_u32(h + 8) != 1) {
+    wfl_set_error("%s: not a loop file (version 1)", path);
+    return WFL_ERR_ARG;
+  }
+  if (Nl) *Nl = (int)get_u32(h + 12);
+  if (N) *N = (int)get_u32(h + 16);
+  if (config_id) *config_id = get_u32(h + 20);
+  return WFL_OK;
+}
+
+int wfl_loops_read(const char *path, float *x, float *y, int Nl, int N) {
+  if (!path || !x || !y || Nl < 1 || N < 1) { wfl_set_error("wfl_loops_read: bad arguments"); return WFL_ERR_ARG; }
+  int fNl = 0, fN = 0;
+  int rc = wfl_loops_read_header(path, &fNl, &fN, NULL);
+  if (rc) return rc;
+  if (fNl != Nl || fN != N) { wfl_set_error("%s holds %d x %d loops, not %d x %d", path, fNl, fN, Nl, N); return WFL_ERR_ARG; }
+  FILE *fp = fopen(path, "rb");
+  if (!fp) { wfl_set_error("cannot open %s", path); return WFL_ERR_ARG; }
+  unsigned char h[40];
+  const size_t n = (size_t)Nl * N;
+  int ok = fread(h, 1, sizeof h, fp) == sizeof h && fread(x, sizeof(float), n, fp) == n &&
+           fread(y, sizeof(float), n, fp) == n;
+  fclose(fp);
+  if (!ok) { wfl_set_error("%s: short read", path); return WFL_ERR_ARG; }
+  if (loops_hash(x, y, n) != get_u64(h + 24)) { wfl_set_error("%s: content hash mismatch", path); return WFL_ERR_ARG; }
   return WFL_OK;
 }
 

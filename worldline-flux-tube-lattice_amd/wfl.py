@@ -88,6 +88,7 @@ EXPORTS = [
     "EV", "EVMean", "Tfunc", "integrateT", "getwl", "Exact", "MagField", "smTscal", "smTferm",
     "rhogrand", "testvalrho", "transformRho", "integrateRhotoa", "integrateRho", "PeriodicCA", "FixFluxCA",
     "ClassAction", "EAction", "wfl_eaction", "wfl_set_verbosity",
+    "wfl_loops_write", "wfl_loops_read_header", "wfl_loops_read",
 ]
 
 
@@ -107,6 +108,9 @@ def _load():
         "wfl_loops_destroy": (None, [P]),
         "wfl_gen_loops": (I, [C.c_uint32, I, I, P, P]),
         "wfl_read_text_loops": (I, [C.c_char_p, I, I, P, P]),
+        "wfl_loops_write": (I, [C.c_char_p, P, P, I, I, C.c_uint32]),
+        "wfl_loops_read_header": (I, [C.c_char_p, P, P, P]),
+        "wfl_loops_read": (I, [C.c_char_p, P, P, I, I]),
         "wfl_plan_create": (I, [P, P, P]),
         "wfl_plan_run": (I, [P, P]),
         "wfl_plan_graph_run": (I, [P, P]),
@@ -184,6 +188,23 @@ def gen_loops(config_id: int, Nl: int, N: int):
     y = np.empty((Nl, N), np.float32)
     check(lib.wfl_gen_loops(config_id, Nl, N, _ptr(x), _ptr(y)))
     return x, y
+
+
+def write_loops(path, x, y, config_id=0):
+    """Store an SoA ensemble as a binary loop file (header + hash)."""
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    check(lib.wfl_loops_write(str(path).encode(), _ptr(x), _ptr(y), x.shape[0], x.shape[1], config_id))
+
+
+def read_loops(path):
+    """Read a binary loop file; returns (x, y, config_id), hash-verified."""
+    Nl, N, cid = C.c_int(), C.c_int(), C.c_uint32()
+    check(lib.wfl_loops_read_header(str(path).encode(), C.addressof(Nl), C.addressof(N), C.addressof(cid)))
+    x = np.empty((Nl.value, N.value), np.float32)
+    y = np.empty((Nl.value, N.value), np.float32)
+    check(lib.wfl_loops_read(str(path).encode(), _ptr(x), _ptr(y), Nl.value, N.value))
+    return x, y, cid.value
 
 
 def spline_build(fli, rho2, quirk=False):

@@ -346,6 +346,8 @@ def run_config(wfl, args, mode, cfg):
     gs_m, _ = mixed.read()
     out["nonfinite_loops_mixed"] = mixed.nonfinite()
     out["accurate_node_tuples"] = mixed.accurate_tuples
+    if cfg.get("splines"):
+        out["uniform_grid_splines"] = mixed.uniform_splines
     mixed.close()
     fl = algorithmic_flops(cfg)
     out.update({"value": pe / ms, "ms_per_step": ms * 1e3, "kernel_ms": km * 1e3,
@@ -353,7 +355,11 @@ def run_config(wfl, args, mode, cfg):
     gs_d = None
     if not args.no_fp64:
         fp64 = r.plan(wfl.FP64)
-        fp64.set_config(*out["launch_config"].values())   # the same launch shape as the mixed run
+        if km < 0.05:
+            # a small workload: the FP64 plan gets its own measured launch too
+            out["fp64_autotune_choice"] = fp64.autotune()
+        else:
+            fp64.set_config(*out["launch_config"].values())   # the same launch shape as the mixed run
         ms64, k64 = r.timed(fp64, args.fp64_steps, 0, "fp64")
         gs_d, _ = fp64.read()
         fp64.close()

@@ -163,6 +163,11 @@ int wfl_plan_ngroups(const wfl_plan *p);
  * are small (sigma sqrt(T) < 1/2, sigma^2 = log2(e)/l2) and can reach the tube
  * core, where the plain quotient (1 - 2^-u)/u loses relative accuracy. */
 int wfl_plan_accurate_tuples(const wfl_plan *p);
+/* 1 when a mixed, physical-mode, spline-only plan evaluates its tables by
+ * direct indexing: every table's knots lie on a uniform grid in
+ * r = sqrt(rho^2) (the layout of periodic.c:129-151), checked to 1e-4 of
+ * the spacing; 0 when the kernels use the exact interval search. */
+int wfl_plan_uniform_splines(const wfl_plan *p);
 /* Device pointer of the group sums (for callers that keep results on device). */
 const double *wfl_plan_gsum_device(const wfl_plan *p);
 /* Number of non-finite per-loop values met in the last run (diagnostic). */

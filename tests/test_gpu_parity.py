@@ -33,7 +33,8 @@ def relerr(a, b):
     return np.max(np.abs(a - b)) / scale
 
 
-def run_case(wfl, orc, loops, geoms, ogeoms, mode, nsub, spinor, prec, mapping, tup, splines=(), group=32):
+def run_case(wfl, orc, loops, geoms, ogeoms, mode, nsub, spinor, prec, mapping, tup, splines=(), group=32,
+             on_plan=None):
     x, y, L = loops
     tg, tcx, tcy, tT = tup
     b = wfl.Batch(list(geoms), tg, tcx, tcy, tT, mode=mode, precision=prec, nsub=nsub, spinor=spinor,
@@ -44,6 +45,8 @@ def run_case(wfl, orc, loops, geoms, ogeoms, mode, nsub, spinor, prec, mapping, 
         plan.run()
         gs, pl = plan.read()
         nonfinite = plan.nonfinite()
+        if on_plan is not None:
+            on_plan(plan)
     finally:
         plan.close()
     rgs, rpl = orc.eval_batch(ogeoms, mode, nsub, spinor, x, y, group, tg, tcx, tcy, tT)
@@ -140,6 +143,38 @@ def test_spline_profile(gpu, orc, loops, prec, tol, mode, mapping):
         e1, e2 = relerr(pl, rpl), relerr(gs, rgs)
         print(f"  spinor {spinor}: perloop {e1:.2e} gsum {e2:.2e}")
         assert e1 < tol and e2 < tol
+
+
+@pytest.mark.parametrize("mapping", [1, 2])
+@pytest.mark.parametrize("nsub", [2, 4])
+def test_spline_uniform_grid(gpu, orc, loops, mapping, nsub):
+    """Mixed physical-mode spline-only plans: two tables with knots uniform
+    in rho (512 knots to rho 20 and 64 to rho 8) take the direct-index
+    kernels; adding a table with irregular knots sends the plan to the exact
+    interval search.  Both match the oracle (which bisects); x_cm on the
+    axis puts nodes below the first knot (the line row) and large T beyond
+    the last (the constant row)."""
+    k1, c1 = gaussian_spline(gpu)
+    k2, c2 = gaussian_spline(gpu, n=64, rho_max=8.0, l2=2.0)
+    k3, c3 = gaussian_spline(gpu, n=300, rho_max=12.0, l2=0.5, rng=np.random.default_rng(3))
+    parts = [tuples([0.0, 0.9], [0.004, 0.3, 2.0, 40.0], geom=g) for g in range(3)]
+    tabs = [(k1, c1), (k2, c2), (k3, c3)]
+    geoms = [gpu.Geometry(kind=gpu.SPLINE, F=1.0, l2=1.0, a=1.0, spline_table=0),
+             gpu.Geometry(kind=gpu.SPLINE, F=2.0, l2=2.0, a=1.0, spline_table=1),
+             gpu.Geometry(kind=gpu.SPLINE, F=0.7, l2=0.5, a=1.0, spline_table=2)]
+    og = [orc.Geom(orc.SPLINE, F=1.0, l2=1.0, tubedist=1.0, knots=k1, coefs=c1),
+          orc.Geom(orc.SPLINE, F=2.0, l2=2.0, tubedist=1.0, knots=k2, coefs=c2),
+          orc.Geom(orc.SPLINE, F=0.7, l2=0.5, tubedist=1.0, knots=k3, coefs=c3)]
+    for ngeom, want_uni in ((2, 1), (3, 0)):
+        tup = tuple(np.concatenate([p[k] for p in parts[:ngeom]]) for k in range(4))
+        seen = []
+        for spinor in (0, 1):
+            gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms[:ngeom], og[:ngeom], 1, nsub, spinor, 1, mapping,
+                                        tup, splines=tabs[:ngeom], on_plan=lambda p: seen.append(p.uniform_splines))
+            e1, e2 = relerr(pl, rpl), relerr(gs, rgs)
+            print(f"  tables {ngeom} spinor {spinor}: perloop {e1:.2e} gsum {e2:.2e}")
+            assert e1 < 2e-5 and e2 < 1e-5
+        assert seen == [want_uni, want_uni]
 
 
 @pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 5e-5)])

@@ -346,14 +346,42 @@ static double unit_scale(const wfl_geometry& g, bool scaled) {
 // can place in b.  The table is "fast" when the interval of the highest such
 // u is at most s + 2 for every bucket; nb is the smallest of n/2, n, 2n, 4n,
 // 8n (at most 16384) for which it is, else 2n with the scan.
+// Uniform-grid rows (FP32 physical mode, tables whose knots lie on
+// r_j = r0 + j h with r0 <= h), at the front of the blob: the n + 1
+// coefficient rows, then their n + 1 left knots as floats (0, k_0, ...,
+// k_{n-2}, k_{n-1}) padded to whole float4.  uni_f4 is the size of that
+// prefix, the part the uniform-grid kernels stage in LDS.
 struct SplineTab {
   int cf, rec, kn, n, nb, fast;
   float r0, inv_w, k0, kl;
+  int uni, ucf, ukn;
+  float uinv_h, uoff;
 };
 struct SplineBlob {
   std::vector<float4> data;
   std::vector<SplineTab> tab;
+  int uni_f4 = 0;        // uniform-grid rows of all tables (a prefix of data)
+  bool all_uni = false;  // every table has them
 };
+
+// Knots on a uniform grid in r = sqrt(rho^2) that starts at or below one
+// spacing: with inv_h = fl32(1/h) and off = fl32(1 - r0 inv_h), every knot
+// k_j has sqrt(k_j) inv_h + off within 1e-4 of j + 1.
+static bool spline_uniform(const float* k, int n, SplineTab& st) {
+  if (n < 3 || !(k[0] >= 0.0f)) return false;
+  const double r0 = sqrt((double)k[0]), r1 = sqrt((double)k[n - 1]);
+  const double h = (r1 - r0) / (double)(n - 1);
+  if (!(h > 0.0) || r0 > h * (1.0 + 1e-4)) return false;
+  const float inv_h = (float)(1.0 / h);
+  const float off = (float)(1.0 - r0 * (double)inv_h);
+  for (int j = 0; j < n; ++j) {
+    const double t = sqrt((double)k[j]) * (double)inv_h + (double)off;
+    if (!(fabs(t - (double)(j + 1)) <= 1e-4)) return false;
+  }
+  st.uinv_h = inv_h;
+  st.uoff = off;
+  return true;
+}
 
 // largest j <= n - 2 with k[j] <= v, 0 if none
 static int spline_interval_host(const float* k, int n, double v) {
@@ -381,9 +409,9 @@ static bool spline_buckets(const float* k, int n, int nb, float r0, float inv_w,
 }
 
 static void build_spline_blob(int ntab, const float* const* rho2, const float* const* coefs, const int* n, int mode,
-                              SplineBlob& sb) {
+                              SplineBlob& sb, bool want_uniform = false) {
   sb = SplineBlob();
-  int rows = 0, btot = 0, ktot = 0;
+  int rows = 0, btot = 0, ktot = 0, urows = 0;
   std::vector<std::vector<int>> starts(ntab);
   for (int t = 0; t < ntab; ++t) {
     const int nt = n[t];
@@ -411,6 +439,14 @@ static void build_spline_blob(int ntab, const float* const* rho2, const float* c
       st.inv_w = (float)(st.nb / span);
       spline_buckets(k, nt, st.nb, st.r0, st.inv_w, margin_of(st.nb), starts[t]);
     }
+    st.uni = (want_uniform && mode == WFL_MODE_PHYSICAL && spline_uniform(k, nt, st)) ? 1 : 0;
+    st.ucf = -1;
+    st.ukn = -1;
+    if (st.uni) {
+      st.ucf = urows;
+      st.ukn = 4 * (urows + nt + 1);
+      urows += nt + 1 + (nt + 1 + 3) / 4;
+    }
     st.cf = rows;
     rows += nt + 1;
     st.rec = btot;
@@ -420,8 +456,10 @@ static void build_spline_blob(int ntab, const float* const* rho2, const float* c
     sb.tab.push_back(st);
   }
   const size_t kn_f4 = ((size_t)ktot + 3) / 4;
-  sb.data.assign((size_t)rows + btot + kn_f4 + 1, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
-  float4* d = sb.data.data();
+  sb.data.assign((size_t)urows + rows + btot + kn_f4 + 1, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+  sb.uni_f4 = urows;
+  sb.all_uni = ntab > 0;
+  float4* d = sb.data.data() + urows;      // the exact-lookup part follows the uniform rows
   float* kn = (float*)(d + rows + btot);
   const float inf = __builtin_inff();
   for (int t = 0; t < ntab; ++t) {
@@ -435,6 +473,15 @@ static void build_spline_blob(int ntab, const float* const* rho2, const float* c
       d[st.cf] = make_float4(0.0f, slope, 0.0f, 0.0f);
       d[st.cf + nt] = make_float4(c[4 * (nt - 1)], 0.0f, 0.0f, 0.0f);
     }
+    sb.all_uni = sb.all_uni && st.uni;
+    if (st.uni) {
+      float4* u = sb.data.data() + st.ucf;
+      float* ul = (float*)sb.data.data() + st.ukn;
+      for (int i = 0; i <= nt; ++i) {
+        u[i] = d[st.cf + i];
+        ul[i] = i == 0 ? 0.0f : k[i - 1];
+      }
+    }
     for (int i = 0; i < nt; ++i) kn[st.kn + i] = k[i];
     for (int b = 0; b < st.nb; ++b) {
       const int s0 = starts[t][b];
@@ -442,8 +489,9 @@ static void build_spline_blob(int ntab, const float* const* rho2, const float* c
       memcpy(&x, &s0, sizeof x);
       d[rows + st.rec + b] = make_float4(x, k[s0], s0 + 1 <= nt - 2 ? k[s0 + 1] : inf, s0 + 2 <= nt - 2 ? k[s0 + 2] : inf);
     }
-    st.rec += rows;                        // float4 offset in the blob
-    st.kn += 4 * (rows + btot);            // float offset in the blob
+    st.cf += urows;                        // float4 offsets in the blob
+    st.rec += urows + rows;
+    st.kn += 4 * (urows + rows + btot);    // float offset in the blob
   }
 }
 
@@ -461,6 +509,10 @@ static void fill_geom(const wfl_geometry& g, const SplineBlob* sb, bool scaled, 
     d.spline_n = t.n; d.spline_nb = t.nb; d.spline_fast = t.fast;
     d.spline_r0 = t.r0; d.spline_inv_w = t.inv_w;
     d.spline_k0 = t.k0; d.spline_kl = t.kl;
+    d.spline_ucf = t.ucf; d.spline_ukn = t.ukn;
+    d.spline_uinv_h = t.uinv_h; d.spline_uoff = t.uoff;
+  } else {
+    d.spline_ucf = -1; d.spline_ukn = -1;
   }
   d.Ff = (float)g.F;       d.F = d.Ff;
   d.l2 = (sg != 1.0) ? WFL_LOG2E / (sg * sg) : (double)(float)g.l2;
@@ -562,6 +614,7 @@ struct wfl_plan_s {
   Task* d_tasks = nullptr;
   float4* d_sblob = nullptr;     // spline blob
   int sblob_f4 = 0, lds_splines = 0;
+  int uni_splines = 0, sblob_uni_f4 = 0;   // uniform-grid spline rows (FP32 physical)
   double* d_gsum = nullptr;
   double* d_perloop = nullptr;
   double* d_wm1 = nullptr;
@@ -697,7 +750,7 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
     for (int s = 0; s < b->nspline; ++s) {
       kp.push_back(b->splines[s].rho2); cp.push_back(b->splines[s].coefs); np.push_back(b->splines[s].n);
     }
-    build_spline_blob(b->nspline, kp.data(), cp.data(), np.data(), b->mode, sblob);
+    build_spline_blob(b->nspline, kp.data(), cp.data(), np.data(), b->mode, sblob, b->precision == WFL_MIXED);
   }
   std::vector<DevGeom> geoms(b->ngeom);
   p->prof_gauss = 1;
@@ -713,6 +766,10 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
   bool all_fast = true;
   for (const SplineTab& t : sblob.tab) all_fast = all_fast && t.fast;
   p->lds_splines = all_spline && all_fast && sblob.data.size() * sizeof(float4) <= kSplineLdsBytes;
+  // FP32 physical spline-only plans on uniform-grid tables: direct-index rows in LDS
+  p->uni_splines = all_spline && sblob.all_uni && (size_t)sblob.uni_f4 * sizeof(float4) <= kSplineLdsBytes;
+  p->sblob_uni_f4 = sblob.uni_f4;
+  if (p->uni_splines) p->lds_splines = 1;
   // tuples: FP32 kernel-unit values and the effective doubles
   p->tups.resize(b->ntup);
   for (int t = 0; t < b->ntup; ++t) {
@@ -820,6 +877,7 @@ static void fill_launch(const wfl_plan* p, WflLaunch& a, hipStream_t s) {
   a.lanes = p->d_lanes; a.tasks = p->d_tasks; a.ntasks = p->ntasks;
   a.nslots = p->mapping == 1 ? p->ngroups * p->cpg : p->ngroups;
   a.sblob = p->d_sblob; a.sblob_f4 = p->sblob_f4; a.lds_splines = p->lds_splines;
+  a.uni_splines = p->uni_splines; a.sblob_uni_f4 = p->sblob_uni_f4;
   a.prof_gauss = p->prof_gauss; a.spinor = p->b.spinor ? 1 : 0; a.mode = p->b.mode; a.nsub = p->b.nsub;
   a.split = p->mapping == 3;
   a.gsum = (p->mapping == 1 && p->cpg > 1) ? p->d_gpart : p->d_gsum;
@@ -930,7 +988,7 @@ extern "C" int wfl_plan_autotune(wfl_plan* p, void* stream, int* chosen) {
   memset(&key, 0, sizeof key);
   key.ntup = p->b.ntup; key.Nl = p->L->Nl; key.N = p->L->N; key.precision = p->b.precision;
   key.spinor = p->b.spinor; key.mode = p->b.mode; key.nsub = p->b.nsub; key.prof_gauss = p->prof_gauss;
-  key.lds_splines = p->lds_splines;
+  key.lds_splines = p->lds_splines + 2 * p->uni_splines;
   key.ntasks1 = p->ntasks1;
   {
     bool hit = false;
@@ -1055,6 +1113,7 @@ extern "C" int wfl_plan_accurate_tuples(const wfl_plan* p) {
   for (const DevTuple& t : p->tups) n += t.acc ? 1 : 0;
   return n;
 }
+extern "C" int wfl_plan_uniform_splines(const wfl_plan* p) { return p ? p->uni_splines : 0; }
 extern "C" const double* wfl_plan_gsum_device(const wfl_plan* p) { return p ? p->d_gsum : nullptr; }
 extern "C" int wfl_plan_mapping(const wfl_plan* p) { return p ? p->mapping : 0; }
 
@@ -1137,7 +1196,7 @@ extern "C" int wfl_internal_ev_launch(const void* dev_aos, int Nl, int N, const 
       if (!(kn[i] > kn[i - 1])) { wfl_set_error("EV: spline knots not increasing at %d", i); return WFL_ERR_ARG; }
     const float* kp = kn.data();
     const float* cp = cf.data();
-    build_spline_blob(1, &kp, &cp, &nspline, mode, sb);
+    build_spline_blob(1, &kp, &cp, &nspline, mode, sb, precision == WFL_MIXED);
     if (sb.data.size() > g_ev.sblob_cap) {
       if (g_ev.d_sblob) (void)hipFree(g_ev.d_sblob);
       g_ev.d_sblob = nullptr; g_ev.sblob_cap = 0;
@@ -1167,6 +1226,9 @@ extern "C" int wfl_internal_ev_launch(const void* dev_aos, int Nl, int N, const 
   if (g->kind == WFL_SPLINE) {
     a.sblob = g_ev.d_sblob; a.sblob_f4 = (int)sb.data.size();
     a.lds_splines = sb.tab[0].fast && sb.data.size() * sizeof(float4) <= kSplineLdsBytes;
+    a.uni_splines = sb.all_uni && (size_t)sb.uni_f4 * sizeof(float4) <= kSplineLdsBytes;
+    a.sblob_uni_f4 = sb.uni_f4;
+    if (a.uni_splines) a.lds_splines = 1;
   }
   a.prof_gauss = g->kind == WFL_GAUSSIAN; a.spinor = fermion == 1; a.mode = mode; a.nsub = nsub;
   a.ws_scal = (float4*)ws_scal_d; a.ws_ferm = fermion == 1 ? (float4*)ws_ferm_d : nullptr;

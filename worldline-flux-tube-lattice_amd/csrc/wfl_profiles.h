@@ -16,6 +16,9 @@ enum { P_STEP = 0, P_SMOOTH = 1, P_QUADRATIC = 2, P_GAUSSIAN = 3,
 enum { M_REFERENCE = 0, M_PHYSICAL = 1 };
 // kernel tag for "profile chosen at run time from DevGeom::kind"
 enum { P_ANY = 99 };
+// Kernel-only profile: a spline whose knots lie on a uniform grid in
+// r = sqrt(rho^2), evaluated in FP32 physical mode by direct indexing.
+enum { P_SPLINE_U = 98 };
 
 // One geometry as the kernels see it.  Every double field holds a value
 // that is exactly representable in FP32.
@@ -33,6 +36,8 @@ struct DevGeom {
   int spline_n, spline_nb, spline_fast;             // knots, buckets, 1: every interval within 2 of its bucket start
   float spline_r0, spline_inv_w;                    // first bucket edge and 1/width in sqrt(rho^2)
   float spline_k0, spline_kl;                       // first and last knot
+  int spline_ucf, spline_ukn;                       // uniform-grid rows (float4 offset, -1: none), their left knots (float offset)
+  float spline_uinv_h, spline_uoff;                 // row of u: trunc(min(sqrt(u) uinv_h + uoff, n))
   double F, l2, inv_l2, a, lmin, q1, q2, lattice;   // effective values (FP64 kernels)
   float Ff, l2f, inv_l2f, af, lminf, q1f, q2f;      // FP32 generic-profile values
   float latticef;                                   // lattice spacing in FP32 kernel units
@@ -254,6 +259,9 @@ struct SplineRef {
   const float* kn;        // knots rho^2
   int n, nb, fast;
   float r0, inv_w, k0, kl;
+  const float4* ucf;      // uniform-grid rows
+  const float* ukn;       // their left knots
+  float uinv_h, uoff;
 };
 
 __device__ __forceinline__ SplineRef spline_ref(const float4* blob, const DevGeom& g) {
@@ -268,7 +276,33 @@ __device__ __forceinline__ SplineRef spline_ref(const float4* blob, const DevGeo
   s.inv_w = g.spline_inv_w;
   s.k0 = g.spline_k0;
   s.kl = g.spline_kl;
+  s.ucf = blob + (g.spline_ucf > 0 ? g.spline_ucf : 0);
+  s.ukn = (const float*)blob + (g.spline_ukn > 0 ? g.spline_ukn : 0);
+  s.uinv_h = g.spline_uinv_h;
+  s.uoff = g.spline_uoff;
   return s;
+}
+
+// Uniform-grid spline, FP32 physical mode: g(u) = f(u)/u and f'(u) from one
+// row in LDS, with no search.  The knots lie on r_j = r0 + j h with
+// r0 <= h (checked on the host to 1e-4 h), so row i = trunc((sqrt(u) - r0)/h) + 1,
+// at most n, is interval i - 1; row 0 is the line s u below the first knot
+// and row n the constant past the last (1 - r0/h > -1, so no lower clamp).
+// Row i holds the coefficients (c0, c1, c2, c3) of the exact lookup's row i
+// (16 B, dense) and, in an array of its own, the row's left knot (0 for
+// row 0, k_{n-1} for row n).  Near a knot, rounding of sqrt(u) may pick
+// the neighbouring interval: interior intervals join with two continuous
+// derivatives and the line with a continuous value, so the difference is at
+// the FP32 rounding level.  u > 0 always holds here (the FP32 kernels add
+// 1e-30).
+__device__ __forceinline__ float spline_u_g(float u, const SplineRef& sp, int n, bool need_fp, float& fp) {
+  const float t = fminf(fmaf(__builtin_amdgcn_sqrtf(u), sp.uinv_h, sp.uoff), (float)n);
+  const int i = (int)t;
+  const float4 c = sp.ucf[i];
+  const float dx = u - sp.ukn[i];
+  const float f = fmaf(dx, fmaf(dx, fmaf(dx, c.w, c.z), c.y), c.x);
+  fp = need_fp ? fmaf(dx, fmaf(dx, 3.0f * c.w, 2.0f * c.z), c.y) : 0.0f;
+  return f * __builtin_amdgcn_rcpf(u);
 }
 
 // Interval j (0 <= j <= n - 2) of u and its left knot; exact for
@@ -369,6 +403,7 @@ __device__ void profile_eval(int kind, const GP<R>& g, R u, int mode, bool need_
 template <class R, int PROF>
 __device__ __forceinline__ R generic_g(int kind_rt, const GP<R>& g, R u, int mode, bool need_fp,
                                        const SplineRef& sp, R& fp) {
+  if constexpr (PROF == P_SPLINE_U && sizeof(R) == 4) return spline_u_g(u, sp, sp.n, need_fp, fp);
   const int kind = PROF == P_ANY ? kind_rt : PROF;
   R f;
   if (mode == M_PHYSICAL) {

@@ -80,6 +80,7 @@ EXPORTS = [
     "wfl_loops_destroy", "wfl_gen_loops", "wfl_read_text_loops",
     "wfl_plan_create", "wfl_plan_run", "wfl_plan_run_timed", "wfl_plan_autotune", "wfl_plan_config",
     "wfl_plan_set_config", "wfl_plan_graph_run", "wfl_plan_read", "wfl_plan_ngroups", "wfl_plan_accurate_tuples",
+    "wfl_plan_uniform_splines",
     "wfl_plan_gsum_device", "wfl_plan_nonfinite", "wfl_plan_mapping", "wfl_plan_destroy",
     "wfl_eval_batch", "wfl_group_ev_minus1", "wfl_spline_build", "wfl_set_profile",
     "wfl_set_numerics", "wfl_ev_reset", "wfl_ev_launches", "wfl_logsimpson", "wfl_effective_T",
@@ -121,6 +122,7 @@ def _load():
         "wfl_plan_read": (I, [P, P, P]),
         "wfl_plan_ngroups": (I, [P]),
         "wfl_plan_accurate_tuples": (I, [P]),
+        "wfl_plan_uniform_splines": (I, [P]),
         "wfl_plan_gsum_device": (P, [P]),
         "wfl_plan_nonfinite": (I, [P, P]),
         "wfl_plan_mapping": (I, [P]),
@@ -329,6 +331,7 @@ class Plan:
         check(lib.wfl_plan_create(C.byref(self.h), C.byref(self._b), loops.h))
         self.ngroups = lib.wfl_plan_ngroups(self.h)
         self.accurate_tuples = lib.wfl_plan_accurate_tuples(self.h)
+        self.uniform_splines = lib.wfl_plan_uniform_splines(self.h)
         self.mapping = lib.wfl_plan_mapping(self.h)
         self.ntup = len(batch.tup_geom)
 

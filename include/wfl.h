@@ -126,6 +126,9 @@ int wfl_gen_loops(uint32_t config_id, int Nl, int N, float *x, float *y);
 /* Text reader for the reference's worldline files (getwl.c:6-40): per loop
  * N x values then N y values.  Writes SoA x, y. */
 int wfl_read_text_loops(const char *path, int Nl, int N, float *x, float *y);
+/* Writes the same text layout (getwl.c:18-28), 9 significant digits, so
+ * that wfl_read_text_loops and getwl read the floats back exactly. */
+int wfl_write_text_loops(const char *path, const float *x, const float *y, int Nl, int N);
 /* Binary loop files (SoA float32 with a header: magic "WFLLOOP1", version,
  * Nl, N, generator config id, FNV-1a 64 hash of the data).  read verifies
  * the shape and the hash (WFL_ERR_ARG on a mismatch). */

@@ -527,6 +527,25 @@ int wfl_read_text_loops(const char *path, int Nl, int N, float *x, float *y) {
   return WFL_OK;
 }
 
+/* The text layout getwl.c:18-28 reads: per loop one line of N x values and
+ * one of N y values, each printed with 9 significant digits (enough for a
+ * float to read back bit-exact). */
+int wfl_write_text_loops(const char *path, const float *x, const float *y, int Nl, int N) {
+  if (!path || !x || !y || Nl < 1 || N < 1) { wfl_set_error("wfl_write_text_loops: bad arguments"); return WFL_ERR_ARG; }
+  FILE *fp = fopen(path, "w");
+  if (!fp) { wfl_set_error("cannot create %s", path); return WFL_ERR_ARG; }
+  int ok = 1;
+  for (int l = 0; l < Nl && ok; ++l) {
+    const float *rows[2] = {x + (size_t)l * N, y + (size_t)l * N};
+    for (int c = 0; c < 2 && ok; ++c)
+      for (int j = 0; j < N && ok; ++j)
+        ok = fprintf(fp, j + 1 < N ? "%.9g " : "%.9g\n", (double)rows[c][j]) > 0;
+  }
+  ok = (fclose(fp) == 0) && ok;
+  if (!ok) { wfl_set_error("%s: write failed", path); return WFL_ERR_ARG; }
+  return WFL_OK;
+}
+
 /* Binary loop file: a 40-byte little-endian header, then x[Nl*N] and
  * y[Nl*N] as float32 (the engine's SoA layout).
  *   char magic[8] = "WFLLOOP1"; uint32 version = 1, Nl, N, config_id;

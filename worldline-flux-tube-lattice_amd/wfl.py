@@ -89,7 +89,7 @@ EXPORTS = [
     "EV", "EVMean", "Tfunc", "integrateT", "getwl", "Exact", "MagField", "smTscal", "smTferm",
     "rhogrand", "testvalrho", "transformRho", "integrateRhotoa", "integrateRho", "PeriodicCA", "FixFluxCA",
     "ClassAction", "EAction", "wfl_eaction", "wfl_set_verbosity",
-    "wfl_loops_write", "wfl_loops_read_header", "wfl_loops_read",
+    "wfl_loops_write", "wfl_loops_read_header", "wfl_loops_read", "wfl_write_text_loops",
 ]
 
 
@@ -110,6 +110,7 @@ def _load():
         "wfl_gen_loops": (I, [C.c_uint32, I, I, P, P]),
         "wfl_read_text_loops": (I, [C.c_char_p, I, I, P, P]),
         "wfl_loops_write": (I, [C.c_char_p, P, P, I, I, C.c_uint32]),
+        "wfl_write_text_loops": (I, [C.c_char_p, P, P, I, I]),
         "wfl_loops_read_header": (I, [C.c_char_p, P, P, P]),
         "wfl_loops_read": (I, [C.c_char_p, P, P, I, I]),
         "wfl_plan_create": (I, [P, P, P]),
@@ -197,6 +198,21 @@ def write_loops(path, x, y, config_id=0):
     x = np.ascontiguousarray(x, np.float32)
     y = np.ascontiguousarray(y, np.float32)
     check(lib.wfl_loops_write(str(path).encode(), _ptr(x), _ptr(y), x.shape[0], x.shape[1], config_id))
+
+
+def write_text_loops(path, x, y):
+    """Store an SoA ensemble in the text layout getwl.c reads."""
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    check(lib.wfl_write_text_loops(str(path).encode(), _ptr(x), _ptr(y), x.shape[0], x.shape[1]))
+
+
+def read_text_loops(path, Nl, N):
+    """Read Nl loops of N points from the getwl.c text layout."""
+    x = np.empty((Nl, N), np.float32)
+    y = np.empty((Nl, N), np.float32)
+    check(lib.wfl_read_text_loops(str(path).encode(), Nl, N, _ptr(x), _ptr(y)))
+    return x, y
 
 
 def read_loops(path):

@@ -21,6 +21,9 @@ dominant kernel and a CPU baseline (the FP64 C oracle on a bounded sample).
       persistent launch, autotuned and replayed as a hipGraph
   c5  65536 x 8192 loops, 256 rho x 128 T, 3 x 3 images at spacing 2;
       ensemble standard error against the precision error (--scale shrinks)
+  dropin  the reference's driver flow through EV(): one integrateT() on the
+      5120 x 1024 ensemble with struct Wparams and device float4 loops, FP64
+      and mixed at the reference's numerics (Simpson n = 50, its clamps)
 """
 import argparse
 import json
@@ -380,6 +383,101 @@ def run_config(wfl, args, mode, cfg):
     print(json.dumps(out))
 
 
+class _Hip:
+    """The few HIP runtime calls a C driver makes around EV()."""
+
+    def __init__(self):
+        import ctypes as C
+        self.C = C
+        self.lib = C.CDLL("libamdhip64.so")
+        self.lib.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.lib.hipFree.argtypes = [C.c_void_p]
+
+    def malloc(self, n):
+        p = self.C.c_void_p()
+        if self.lib.hipMalloc(self.C.byref(p), n) != 0:
+            raise SystemExit("hipMalloc failed")
+        return p
+
+    def h2d(self, dst, src):
+        if self.lib.hipMemcpy(dst, src.ctypes.data, src.nbytes, 1) != 0:
+            raise SystemExit("hipMemcpy failed")
+
+
+def bench_dropin(wfl, args):
+    """The reference's own driver flow through the drop-in boundary: one
+    integrateT() (Tint.c:100-175) at rho = 0.5 on the EffAct.c:311 ensemble
+    (5120 loops x 1024 points, float4 AoS in device memory, struct Wparams),
+    Gaussian tube F = 1, l2 = 1, ng = 40 groups, with the reference's numerics
+    (reference clamps, Simpson n = 50) in FP64 and in mixed precision.  Each
+    EV() launch evaluates all 5120 loops at one (x_cm, T); Tfunc's 40 calls
+    per T reuse it.  CPU: the FP64 C oracle on 256 loops at T = 1, scaled to
+    one launch."""
+    C = __import__("ctypes")
+    Nl, Nppl, ng, rho = 5120, 1024, 40, 0.5
+    fermion = 1 if args.species == "spinor" else 0
+    hip = _Hip()
+    x, y = wfl.gen_loops(1, Nl, Nppl)
+    aos = np.zeros((Nl * Nppl, 4), np.float32)
+    aos[:, 0] = x.ravel()
+    aos[:, 1] = y.ravel()
+    d_wl, d_s, d_f = hip.malloc(aos.nbytes), hip.malloc(Nl * 16), hip.malloc(Nl * 16)
+    hip.h2d(d_wl, aos)
+    hs = np.zeros((Nl, 4), np.float32)
+    hf = np.zeros((Nl, 4), np.float32)
+    p = wfl.Wparams()
+    p.Nl, p.Nppl, p.F, p.l2 = Nl, Nppl, 1.0, 1.0
+    p.xcm.x, p.xcm.y = rho, 0.0
+    p.Wsscal_h, p.Wsscal_d = hs.ctypes.data, d_s.value
+    p.Wsferm_h, p.Wsferm_d = hf.ctypes.data, d_f.value
+    p.worldlines = d_wl.value
+    p.nBlocks, p.nThreads, p.ng, p.fermion = (Nl + 511) // 512, 512, ng, fermion
+    wfl.set_profile(wfl.Geometry(kind=wfl.GAUSSIAN, F=1.0, l2=1.0))
+    out = {"workload": f"dropin_integrateT_{'spinor' if fermion else 'scalar'}_{Nl}x{Nppl}_rho{rho}",
+           "n_gpus": 1, "data": "synthetic (seeded Philox unit loops)"}
+    sys.stdout.flush()
+    saved = os.dup(1)
+    try:
+        os.dup2(2, 1)        # integrateT prints its Leffvsrho: lines; keep stdout for the JSON line
+        for prec, tag in ((wfl.FP64, "fp64"), (wfl.MIXED, "mixed")):
+            wfl.set_numerics(wfl.MODE_REFERENCE, prec, 50)
+            wfl.lib.wfl_ev_reset()
+            n0 = wfl.lib.wfl_ev_launches()
+            t0 = time.perf_counter()
+            r = wfl.lib.integrateT(ng, C.byref(p))
+            dt = time.perf_counter() - t0
+            launches = wfl.lib.wfl_ev_launches() - n0
+            leff = np.ctypeslib.as_array(C.cast(r, C.POINTER(C.c_double)), shape=(ng,)).copy()
+            out[tag] = {"seconds": dt, "ev_launches": int(launches),
+                        "point_evals_per_s": launches * Nl * Nppl / dt, "leff_group_mean": float(leff.mean())}
+            log(f"dropin {tag}: {dt:.3f} s, {launches} launches")
+    finally:
+        sys.stdout.flush()
+        os.dup2(saved, 1)
+        os.close(saved)
+    out["mixed_over_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+    out["leff_rel_diff_mixed_vs_fp64"] = abs(out["mixed"]["leff_group_mean"] - out["fp64"]["leff_group_mean"]) / \
+        max(abs(out["fp64"]["leff_group_mean"]), 1e-300)
+    for q in (d_wl, d_s, d_f):
+        hip.lib.hipFree(q)
+    wfl.lib.wfl_ev_reset()
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+        g = orc.Geom(orc.GAUSSIAN, F=1.0, l2=1.0, quantize="float")
+        nl = 256
+        t0 = time.perf_counter()
+        orc.eval_batch([g], 0, 50, fermion, x[:nl], y[:nl], nl, np.zeros(1, np.int32), np.array([rho]),
+                       np.zeros(1), np.ones(1), perloop=False, nthreads=cores)
+        dt = (time.perf_counter() - t0) * Nl / nl
+        out["cpu_baseline"] = {"value": Nl * Nppl / dt, "unit": "point-evals/s", "cores": cores, "kind": "port",
+                               "sample": f"FP64 C oracle, first {nl} of {Nl} loops at T = 1, n = 50"}
+        out["cpu_integrateT_estimate_s"] = dt * out["fp64"]["ev_launches"]
+    print(json.dumps(out))
+
+
 def bench_c2(wfl, args, mode):
     cfg = config_c2()
     x, y, loops = load_loops(wfl, cfg)
@@ -452,9 +550,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--fp64-steps", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "dropin"])
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the loop ensemble (c3, c5)")
-    ap.add_argument("--species", default="both", choices=["scalar", "spinor", "both"], help="c3")
+    ap.add_argument("--species", default="both", choices=["scalar", "spinor", "both"], help="c3; dropin: spinor or scalar")
     ap.add_argument("--mode", default="physical", choices=["physical", "reference"])
     ap.add_argument("--mapping", type=int, default=-1,
                     help="0 autotune, 1 lane/tuple, 2 wave/loop, 3 lane/tuple split (default: 1 for c2, 0 otherwise)")
@@ -495,6 +593,8 @@ def main():
     mode = wfl.MODE_PHYSICAL if args.mode == "physical" else wfl.MODE_REFERENCE
     if args.config == "c2":
         return bench_c2(wfl, args, mode)
+    if args.config == "dropin":
+        return bench_dropin(wfl, args)
     if args.config == "c3":
         for sp in {"scalar": [0], "spinor": [1], "both": [0, 1]}[args.species]:
             run_config(wfl, args, mode, config_c3(args.scale, sp))

@@ -92,8 +92,12 @@ def test_ev_matches_oracle_and_reuses_results(setup, orc, fermion):
     assert ev == pytest.approx(orc.evmean(ws, wf, WL, fermion), rel=1e-6)
     assert wfl.lib.wfl_ev_launches() == 1
     WL2 = np.arange(128, 256, dtype=np.int32)
+    keep_s, keep_f = hs.copy(), hf.copy()
+    hs[:] = -7.0                                            # EV overwrites the host buffers every call
+    hf[:] = -7.0
     ev2 = wfl.lib.EV(T, C.byref(p), WL2.ctypes.data)        # second group: no relaunch
     assert wfl.lib.wfl_ev_launches() == 1
+    assert np.array_equal(hs, keep_s) and (not fermion or np.array_equal(hf, keep_f))
     assert ev2 == orc.evmean(hs, hf, WL2, fermion)
     wfl.lib.EV(1.3, C.byref(p), WL.ctypes.data)            # new T: relaunch
     assert wfl.lib.wfl_ev_launches() == 2

@@ -44,6 +44,23 @@ static struct {
   unsigned version;
 } g_evkey;
 
+/* Host copy of the last launch's per-loop outputs: a repeated EV() call with
+ * the same key refills the caller's Wsscal_h/Wsferm_h from here instead of
+ * copying from the device again. */
+static float4 *g_ev_hs = NULL, *g_ev_hf = NULL;
+static size_t g_ev_hcap = 0;
+
+static int ev_cache_reserve(size_t n) {
+  if (n <= g_ev_hcap) return 0;
+  float4 *s = (float4 *)realloc(g_ev_hs, n * sizeof(float4));
+  if (s) g_ev_hs = s;
+  float4 *f = (float4 *)realloc(g_ev_hf, n * sizeof(float4));
+  if (f) g_ev_hf = f;
+  if (!s || !f) return -1;
+  g_ev_hcap = n;
+  return 0;
+}
+
 int wfl_set_profile(const wfl_geometry *g, int nspline) {
   if (!g) { wfl_set_error("wfl_set_profile: NULL"); return WFL_ERR_ARG; }
   if (g->kind < WFL_STEP || g->kind > WFL_FIXFLUX_FF) { wfl_set_error("wfl_set_profile: unknown profile %d", g->kind); return WFL_ERR_ARG; }
@@ -68,6 +85,10 @@ int wfl_set_numerics(int mode, int precision, int nsub) {
 void wfl_ev_reset(void) {
   g_evkey.valid = 0;
   g_ev_launches = 0;
+  free(g_ev_hs);
+  free(g_ev_hf);
+  g_ev_hs = g_ev_hf = NULL;
+  g_ev_hcap = 0;
   wfl_internal_ev_release();
 }
 
@@ -108,15 +129,23 @@ double EV(double T, void *p, int *WLlist) {
     g_evkey.cx = params.xcm.x; g_evkey.cy = params.xcm.y; g_evkey.rtT = rtT;
     g_evkey.F = (float)params.F; g_evkey.l2 = (float)params.l2; g_evkey.version = g_cfg_version;
   }
-  if (wfl_internal_d2h(params.Wsscal_h, params.Wsscal_d, (size_t)params.Nl * sizeof(float4))) {
-    printf("EV(): %s\n", wfl_last_error());
-    g_evkey.valid = 0;
-    return NAN;
-  }
-  if (fermion && wfl_internal_d2h(params.Wsferm_h, params.Wsferm_d, (size_t)params.Nl * sizeof(float4))) {
-    printf("EV(): %s\n", wfl_last_error());
-    g_evkey.valid = 0;
-    return NAN;
+  const size_t nb = (size_t)params.Nl * sizeof(float4);
+  if (hit) {
+    memcpy(params.Wsscal_h, g_ev_hs, nb);
+    if (fermion) memcpy(params.Wsferm_h, g_ev_hf, nb);
+  } else {
+    if (wfl_internal_d2h(params.Wsscal_h, params.Wsscal_d, nb) ||
+        (fermion && wfl_internal_d2h(params.Wsferm_h, params.Wsferm_d, nb))) {
+      printf("EV(): %s\n", wfl_last_error());
+      g_evkey.valid = 0;
+      return NAN;
+    }
+    if (ev_cache_reserve((size_t)params.Nl) == 0) {
+      memcpy(g_ev_hs, params.Wsscal_h, nb);
+      if (fermion) memcpy(g_ev_hf, params.Wsferm_h, nb);
+    } else {
+      g_evkey.valid = 0;                 /* no host copy: the next call launches again */
+    }
   }
   EVMean(&ev, params.Wsscal_h, params.Wsferm_h, groupsize, WLlist, T, params.fermion);
   return ev;

@@ -282,6 +282,16 @@ class Runner:
                       geom_nloops=cfg.get("geom_nloops"), mapping=mapping)
         return wfl.Plan(b, self.loops)
 
+    def simpson_check(self, tol=1e-6):
+        """Simpson sub-steps the accuracy autotuner asks for on a sample of the
+        workload (256 tuples x 256 loops, FP64), against the config's n."""
+        cfg, wfl = self.cfg, self.wfl
+        b = wfl.Batch(make_geoms(wfl, cfg), *cfg["tup"], mode=self.mode, precision=wfl.FP64, nsub=cfg["nsub"],
+                      spinor=cfg["spinor"], group=GROUP, splines=cfg.get("splines", []),
+                      geom_nloops=cfg.get("geom_nloops"))
+        n, e = wfl.autotune_nsub(b, self.loops, tol, nmax=64, max_tuples=256, max_loops=256)
+        return {"tol": tol, "nsub_chosen": n, "rel_change_at_chosen": e, "nsub_used": cfg["nsub"]}
+
     def timed(self, plan, steps, warmup, label):
         wfl = self.wfl
         for _ in range(warmup):
@@ -369,6 +379,7 @@ def run_config(wfl, args, mode, cfg):
         out["fp64_value"] = pe / ms64
         out["mixed_over_fp64"] = ms64 / ms
         out["accuracy_mixed_vs_fp64"] = accuracy(wfl, cfg, gs_m, gs_d)
+        out["simpson_autotune"] = r.simpson_check()
     loops.close()
     if "lattices" in cfg:
         out["interaction_energy"] = interaction_energy(wfl, cfg, gs_d if gs_d is not None else gs_m)
@@ -529,6 +540,9 @@ def bench_c2(wfl, args, mode):
         result["mixed_over_fp64"] = result["value"] / (pe_step / ms64)
         result["fp64_kernel_ms"] = k64 * 1e3
         result["accuracy_mixed_vs_fp64"] = accuracy(wfl, cfg, gs_mixed, gs64)
+        t0 = time.perf_counter()
+        result["simpson_autotune"] = r.simpson_check()
+        log(f"simpson autotune {result['simpson_autotune']} in {time.perf_counter() - t0:.1f}s")
         if args.dump:
             np.savez_compressed(args.dump, mixed=gs_mixed.sum(axis=1), fp64=gs64.sum(axis=1),
                                 tup=np.stack(cfg["tup"]))

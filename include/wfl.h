@@ -171,6 +171,16 @@ int wfl_plan_accurate_tuples(const wfl_plan *p);
  * r = sqrt(rho^2) (the layout of periodic.c:129-151), checked to 1e-4 of
  * the spacing; 0 when the kernels use the exact interval search. */
 int wfl_plan_uniform_splines(const wfl_plan *p);
+/* Simpson sub-steps chosen by measured accuracy (the batch's own nsub is
+ * ignored): the smallest n in 2, 4, 8, ... <= nmax whose group sums on a
+ * sample of the batch (every k-th tuple, at most max_tuples; the first
+ * max_loops loops, one group) change by at most tol, relative to the
+ * sample's largest |sum of W - 1|, when n is doubled.  *est_err = that
+ * relative change (about the n-point error, Simpson converging as n^-4);
+ * nmax when none passes.  Runs a few small plans; precision and mode are
+ * the batch's. */
+int wfl_autotune_nsub(const wfl_batch *b, wfl_loops *L, double tol, int nmax, int max_tuples, int max_loops,
+                      int *nsub, double *est_err);
 /* Device pointer of the group sums (for callers that keep results on device). */
 const double *wfl_plan_gsum_device(const wfl_plan *p);
 /* Number of non-finite per-loop values met in the last run (diagnostic). */

@@ -54,6 +54,10 @@ def test_argument_errors_are_reported(wfl):
     assert wfl.lib.wfl_loops_create(C.byref(h), x.ctypes.data, x.ctypes.data, 2, 6) == -1
     assert wfl.lib.wfl_gen_loops(1, 2, 6, x.ctypes.data, x.ctypes.data) == -1
     assert wfl.lib.wfl_set_numerics(0, 0, 3) == -1
+    n, e = C.c_int(), C.c_double()
+    cb, keep = b4.c()
+    assert wfl.lib.wfl_autotune_nsub(C.byref(cb), C.c_void_p(1), -1.0, 64, 16, 16, C.byref(n), C.byref(e)) == -1
+    assert wfl.lib.wfl_autotune_nsub(C.byref(cb), None, 1e-6, 64, 16, 16, C.byref(n), C.byref(e)) == -1
     with pytest.raises(wfl.WflError):
         wfl.spline_build([0.0, 1.0, 0.5], [0.0, 1.0, 1.0])
 

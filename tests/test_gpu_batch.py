@@ -106,3 +106,45 @@ def test_autotune_and_graph_replay(gpu, orc, uneven):
         e2, _ = plan.read()
         assert relerr(e2, eager) < 1e-14
     plan.close()
+
+
+def test_autotune_nsub(gpu):
+    """Simpson sub-steps by measured accuracy: tighter tolerances ask for more
+    sub-steps; the chosen n's group sums are within tol of those at 2n and
+    n/2's are not (checked with separate batches).  Loops of size ~sqrt(T)
+    cross a narrow tube's core, where few sub-steps are least accurate."""
+    x, y = gpu.gen_loops(21, 128, 128)
+    L = gpu.Loops(x, y)
+    geoms = [gpu.Geometry(kind=gpu.GAUSSIAN, F=1.0, l2=0.25)]
+    tg = np.zeros(8, np.int32)
+    tcx = np.linspace(0.0, 1.0, 8)
+    tcy = np.zeros(8)
+    tT = np.full(8, 2.0)
+
+    def sums(n):
+        b = gpu.Batch(geoms, tg, tcx, tcy, tT, precision=gpu.FP64, spinor=1, nsub=n, group=128)
+        gs, _ = gpu.eval_batch(b, L)
+        return gs[:, 0, :]
+
+    def rel(a, b):
+        return np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+    chosen = []
+    for tol in (1e-2, 1e-4, 1e-6, 1e-8):
+        b = gpu.Batch(geoms, tg, tcx, tcy, tT, precision=gpu.FP64, spinor=1, nsub=2, group=128)
+        print(f"autotune_nsub tol={tol}", flush=True)
+        n, e = gpu.autotune_nsub(b, L, tol, nmax=256, max_tuples=8, max_loops=128)
+        chosen.append(n)
+        assert n < 256 and e <= tol
+        assert rel(sums(n), sums(2 * n)) == pytest.approx(e, rel=1e-9)
+        if n > 2:
+            assert rel(sums(n // 2), sums(n)) > tol
+    assert chosen == sorted(chosen) and chosen[-1] > chosen[0]
+    # a sample smaller than the ensemble (the first 64 of 128 loops)
+    b = gpu.Batch(geoms, tg, tcx, tcy, tT, precision=gpu.FP64, spinor=1, nsub=2, group=128)
+    n, e = gpu.autotune_nsub(b, L, 1e-6, nmax=256, max_tuples=4, max_loops=64)
+    b64 = [gpu.Batch(geoms, tg[::2], tcx[::2], tcy[::2], tT[::2], precision=gpu.FP64, spinor=1, nsub=m, group=64,
+                     geom_nloops=np.array([64], np.int32)) for m in (n, 2 * n)]
+    g1, g2 = (gpu.eval_batch(bb, L)[0][:, 0, :] for bb in b64)
+    assert rel(g1, g2) == pytest.approx(e, rel=1e-9) and e <= 1e-6
+    L.close()

@@ -1135,6 +1135,81 @@ extern "C" int wfl_eval_batch(const wfl_batch* b, wfl_loops* L, double* gsum, do
   return rc;
 }
 
+// ------------------------------------------------------ Simpson sub-steps
+// The smallest n = 2, 4, 8, ... whose group sums on a sample of the batch
+// (every stride-th tuple, at most max_tuples; the first max_loops loops of
+// each geometry, one group) differ from those at 2n by at most tol relative
+// to the sample's largest |W - 1| sum.  Composite Simpson converges as n^-4,
+// so that difference is 15/16 of the error of the n-point sums.  When no
+// n <= nmax/2 passes, *nsub = nmax and *est_err is the last difference
+// (the estimate for nmax/2; nmax itself is about 16 times closer).
+extern "C" int wfl_autotune_nsub(const wfl_batch* b, wfl_loops* L, double tol, int nmax, int max_tuples,
+                                 int max_loops, int* nsub, double* est_err) {
+  if (!b || !L || !nsub || !(tol > 0.0) || nmax < 2 || max_tuples < 1 || max_loops < 1 || b->ntup < 1) {
+    wfl_set_error("wfl_autotune_nsub: bad arguments");
+    return WFL_ERR_ARG;
+  }
+  int Nl = 0, N = 0;
+  int rc = wfl_loops_shape(L, &Nl, &N);
+  if (rc) return rc;
+  const int nl = std::min(Nl, max_loops);
+  const int stride = (b->ntup + max_tuples - 1) / max_tuples;
+  std::vector<int> tg, gl(b->ngeom);
+  std::vector<double> cx, cy, tT;
+  for (int t = 0; t < b->ntup; t += stride) {
+    tg.push_back(b->tup_geom[t]);
+    cx.push_back(b->tup_cx[t]);
+    cy.push_back(b->tup_cy[t]);
+    tT.push_back(b->tup_T[t]);
+  }
+  for (int g = 0; g < b->ngeom; ++g) gl[g] = b->geom_nloops ? std::min(nl, b->geom_nloops[g]) : nl;
+  wfl_batch s = *b;
+  s.ntup = (int)tg.size();
+  s.tup_geom = tg.data(); s.tup_cx = cx.data(); s.tup_cy = cy.data(); s.tup_T = tT.data();
+  s.geom_nloops = gl.data();
+  s.group = nl;
+  s.want_perloop = 0;
+  s.mapping = 0;
+  std::vector<double> prev, cur, all;
+  double err = HUGE_VAL;
+  int n = 2;
+  for (;;) {
+    s.nsub = n;
+    wfl_plan* p = nullptr;
+    rc = wfl_plan_create(&p, &s, L);
+    if (rc) return rc;
+    // group sums [ntup][ngroups][3]: the sample's loops all sit in group 0
+    const int ng = wfl_plan_ngroups(p);
+    all.assign((size_t)s.ntup * ng * 3, 0.0);
+    rc = wfl_plan_run(p, nullptr);
+    if (!rc) rc = wfl_plan_read(p, all.data(), nullptr);
+    wfl_plan_destroy(p);
+    if (rc) return rc;
+    cur.assign((size_t)s.ntup * 3, 0.0);
+    for (int t = 0; t < s.ntup; ++t)
+      for (int k = 0; k < 3; ++k) cur[(size_t)t * 3 + k] = all[((size_t)t * ng) * 3 + k];
+    if (!prev.empty()) {
+      double dmax = 0.0, vmax = 0.0;
+      for (size_t i = 0; i < cur.size(); ++i) {
+        dmax = std::max(dmax, fabs(cur[i] - prev[i]));
+        vmax = std::max(vmax, fabs(cur[i]));
+      }
+      err = vmax > 0.0 ? dmax / vmax : dmax;
+      if (err <= tol) {
+        *nsub = n / 2;
+        if (est_err) *est_err = err;
+        return WFL_OK;
+      }
+    }
+    if (2 * n > nmax) break;
+    prev = cur;
+    n *= 2;
+  }
+  *nsub = n;
+  if (est_err) *est_err = err;
+  return WFL_OK;
+}
+
 // ------------------------------------------------------ EV() helper
 // Loop copies of the caller's AoS device buffer, reused while the pointer
 // and shape stay the same.

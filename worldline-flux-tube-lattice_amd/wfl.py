@@ -80,7 +80,7 @@ EXPORTS = [
     "wfl_loops_destroy", "wfl_gen_loops", "wfl_read_text_loops",
     "wfl_plan_create", "wfl_plan_run", "wfl_plan_run_timed", "wfl_plan_autotune", "wfl_plan_config",
     "wfl_plan_set_config", "wfl_plan_graph_run", "wfl_plan_read", "wfl_plan_ngroups", "wfl_plan_accurate_tuples",
-    "wfl_plan_uniform_splines",
+    "wfl_plan_uniform_splines", "wfl_autotune_nsub",
     "wfl_plan_gsum_device", "wfl_plan_nonfinite", "wfl_plan_mapping", "wfl_plan_destroy",
     "wfl_eval_batch", "wfl_group_ev_minus1", "wfl_spline_build", "wfl_set_profile",
     "wfl_set_numerics", "wfl_ev_reset", "wfl_ev_launches", "wfl_logsimpson", "wfl_effective_T",
@@ -124,6 +124,7 @@ def _load():
         "wfl_plan_ngroups": (I, [P]),
         "wfl_plan_accurate_tuples": (I, [P]),
         "wfl_plan_uniform_splines": (I, [P]),
+        "wfl_autotune_nsub": (I, [P, P, C.c_double, I, I, I, P, P]),
         "wfl_plan_gsum_device": (P, [P]),
         "wfl_plan_nonfinite": (I, [P, P]),
         "wfl_plan_mapping": (I, [P]),
@@ -406,6 +407,16 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def autotune_nsub(batch: Batch, loops: Loops, tol, nmax=64, max_tuples=256, max_loops=256):
+    """Smallest Simpson n (2, 4, 8, ...) whose sample group sums change by at
+    most tol (relative) when n doubles; returns (n, relative change)."""
+    cb, keep = batch.c()
+    n, e = C.c_int(), C.c_double()
+    check(lib.wfl_autotune_nsub(C.byref(cb), loops.h, float(tol), int(nmax), int(max_tuples), int(max_loops),
+                                C.byref(n), C.byref(e)))
+    return n.value, e.value
 
 
 def eval_batch(batch: Batch, loops: Loops):

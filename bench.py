@@ -11,7 +11,7 @@ Prints one JSON line: mixed-precision throughput, the FP64 engine on the same
 workload, the L_eff / <W> error of mixed against FP64, the roofline of the
 dominant kernel and a CPU baseline (the FP64 C oracle on a bounded sample).
 
---config selects the other survey workloads (one JSON line each):
+--config selects the other workloads (one JSON line each):
   c1  scalar, 512-knot spline of the Gaussian profile, 1024 x 1024 loops,
       32 rho x 64 T; GPU FP64 and mixed, and the CPU oracle at full size
   c3  dense lattice of Gaussian tubes, 5 x 5 images, x_cm on a 32 x 32 grid

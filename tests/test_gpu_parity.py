@@ -177,6 +177,46 @@ def test_spline_uniform_grid(gpu, orc, loops, mapping, nsub):
         assert seen == [want_uni, want_uni]
 
 
+@pytest.mark.parametrize("spinor", [0, 1])
+def test_spline_uniform_grid_at_knots(gpu, orc, loops, spinor):
+    """Loops centred on knots, so that nodes fall on both sides of them,
+    where the FP32 square root can pick the neighbouring row.
+    * Small loops across the first knots of a lambda = 1 table, where the
+      line row meets the first cubic with a continuous value only: within
+      1e-5 of the oracle's bisection.
+    * Loops across the last knot (the constant row) of a lambda = 10 table,
+      whose f still varies there.  At rho = 20 the symmetric gauge sums the
+      loop's small flux from terms ~1e4 larger, so FP32 kernels of either
+      lookup sit near 1e-4; the direct-index kernel is held to the exact
+      search's own error on the same table with one knot moved by 1e-3 of
+      the spacing (not uniform, so the plan takes the exact search)."""
+    def case(knots, coefs, l2, idx, Ts):
+        r = np.sqrt(knots.astype(np.float64))
+        geoms = [gpu.Geometry(kind=gpu.SPLINE, F=1.0, l2=l2, a=1.0, spline_table=0)]
+        og = [orc.Geom(orc.SPLINE, F=1.0, l2=l2, tubedist=1.0, knots=knots, coefs=coefs)]
+        tg, tcx, tcy, tT = tuples([r[j] for j in idx], Ts)
+        seen = []
+        gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, 1, 2, spinor, 1, 1, (tg, tcx, tcy, tT),
+                                    splines=[(knots, coefs)], on_plan=lambda p: seen.append(p.uniform_splines))
+        errs = np.array([relerr(pl[t], rpl[t]) for t in range(len(tg))])
+        for t in range(len(tg)):
+            print(f"  l2 {l2} uniform {seen[0]} x_cm {tcx[t]:.4f} T {tT[t]:.0e}: perloop {errs[t]:.2e}")
+        return seen[0], errs
+
+    k1, c1 = gaussian_spline(gpu, l2=1.0)
+    uni, e = case(k1, c1, 1.0, [0, 1, 2, 5], [1e-4, 1e-3])
+    assert uni == 1 and np.all(e < 1e-5)
+    k2, c2 = gaussian_spline(gpu, l2=100.0)
+    r2 = np.sqrt(k2.astype(np.float64))
+    k3 = k2.copy()
+    k3[3] = np.float32((r2[3] + 1e-3 * (r2[1] - r2[0])) ** 2)
+    c3 = gpu.spline_build(-np.expm1(-k3.astype(np.float64) / 100.0), k3)
+    uni, e_uni = case(k2, c2, 100.0, [508, 511], [0.25, 1.0])
+    exact, e_exact = case(k3, c3, 100.0, [508, 511], [0.25, 1.0])
+    assert uni == 1 and exact == 0
+    assert np.all(e_uni < 2.0 * e_exact + 1e-6), (e_uni, e_exact)
+
+
 @pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 5e-5)])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_spline_irregular_knots_and_mixed_profiles(gpu, orc, loops, prec, tol, mode):

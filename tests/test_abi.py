@@ -49,6 +49,12 @@ def test_argument_errors_are_reported(wfl):
                    np.array([0.0]), np.array([1.0]))
     cb, keep = b4.c()
     assert wfl.lib.wfl_plan_create(C.byref(h), C.byref(cb), C.c_void_p(1)) == -1
+    # an unused table still goes into the plan: it is checked too
+    bad = (np.array([0.0, 1.0, 0.5], np.float32), np.zeros((3, 4), np.float32))
+    b5 = wfl.Batch([wfl.Geometry()], np.array([0]), np.array([0.0]), np.array([0.0]), np.array([1.0]), splines=[bad])
+    cb, keep = b5.c()
+    assert wfl.lib.wfl_plan_create(C.byref(h), C.byref(cb), C.c_void_p(1)) == -1
+    assert b"knots not increasing" in wfl.lib.wfl_last_error()
     # loop shapes: N must be a multiple of 4
     x = np.zeros((2, 6), np.float32)
     assert wfl.lib.wfl_loops_create(C.byref(h), x.ctypes.data, x.ctypes.data, 2, 6) == -1

@@ -586,12 +586,25 @@ static int check_geom(const wfl_geometry& g, int nspline_tables, const wfl_splin
   if ((g.kind == WFL_PERIODIC || g.kind == WFL_FIXFLUX || g.kind == WFL_FIXFLUX_FF) && !(g.a > 0.0)) {
     wfl_set_error("geometry: profile %d needs tubedist a > 0", g.kind); return WFL_ERR_ARG;
   }
-  if (g.kind == WFL_SPLINE) {
-    if (g.spline_table < 0 || g.spline_table >= nspline_tables || !tabs) { wfl_set_error("geometry: bad spline table index %d", g.spline_table); return WFL_ERR_ARG; }
-    const wfl_spline_table& t = tabs[g.spline_table];
-    if (t.n < 2 || !t.rho2 || !t.coefs) { wfl_set_error("spline table %d: need n >= 2 and data", g.spline_table); return WFL_ERR_ARG; }
+  if (g.kind == WFL_SPLINE && (g.spline_table < 0 || g.spline_table >= nspline_tables || !tabs)) {
+    wfl_set_error("geometry: bad spline table index %d", g.spline_table);
+    return WFL_ERR_ARG;
+  }
+  return WFL_OK;
+}
+
+// Every table of a batch goes into the plan's blob, used or not.
+static int check_spline_tables(int nspline_tables, const wfl_spline_table* tabs) {
+  if (nspline_tables < 0 || (nspline_tables > 0 && !tabs)) { wfl_set_error("spline tables missing"); return WFL_ERR_ARG; }
+  for (int s = 0; s < nspline_tables; ++s) {
+    const wfl_spline_table& t = tabs[s];
+    if (t.n < 2 || !t.rho2 || !t.coefs) { wfl_set_error("spline table %d: need n >= 2 and data", s); return WFL_ERR_ARG; }
+    if (!isfinite(t.rho2[0])) { wfl_set_error("spline table %d: knot 0 not finite", s); return WFL_ERR_ARG; }
     for (int i = 1; i < t.n; ++i)
-      if (!(t.rho2[i] > t.rho2[i - 1])) { wfl_set_error("spline table %d: knots not increasing at %d", g.spline_table, i); return WFL_ERR_ARG; }
+      if (!(t.rho2[i] > t.rho2[i - 1]) || !isfinite(t.rho2[i])) {
+        wfl_set_error("spline table %d: knots not increasing at %d", s, i);
+        return WFL_ERR_ARG;
+      }
   }
   return WFL_OK;
 }
@@ -719,6 +732,7 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
     wfl_set_error("tuple arrays missing"); return WFL_ERR_ARG;
   }
   if (b->mapping < 0 || b->mapping > 3) { wfl_set_error("bad mapping %d", b->mapping); return WFL_ERR_ARG; }
+  { int rc = check_spline_tables(b->nspline, b->splines); if (rc) return rc; }
   for (int g = 0; g < b->ngeom; ++g) { int rc = check_geom(b->geoms[g], b->nspline, b->splines); if (rc) return rc; }
   for (int t = 0; t < b->ntup; ++t) {
     if (b->tup_geom[t] < 0 || b->tup_geom[t] >= b->ngeom) { wfl_set_error("tuple %d: geometry index out of range", t); return WFL_ERR_ARG; }

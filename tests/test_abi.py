@@ -224,3 +224,33 @@ def test_generator_cli(wfl, tmp_path, fmt):
         rx, ry, cid = wfl.read_loops(tmp_path / name)
         assert cid == 5
     assert np.array_equal(rx, x) and np.array_equal(ry, y)
+
+
+def test_c_driver_links_and_agrees(wfl, tmp_path):
+    """A C program written like the reference's drivers (intT.h, struct
+    Wparams filled by field) compiles with gcc, links libwfl.so and gets the
+    same values as the ctypes calls: the header's struct layout and the
+    calling convention match the library's."""
+    import subprocess
+    import sys
+    exe = tmp_path / "driver"
+    pkg = os.path.dirname(wfl.LIB_PATH)
+    cmd = ["gcc", "-std=c11", "-O1", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"),
+           "-I", "/opt/rocm/include", os.path.join(ROOT, "tests", "c", "driver.c"), "-L", pkg, "-lwfl",
+           f"-Wl,-rpath,{pkg}", "-lm", "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    vals = dict(line.split("=", 1) for line in r.stdout.splitlines() if "=" in line)
+    assert int(vals["sizeof_Wparams"]) == C.sizeof(wfl.Wparams)
+    p = wfl.Wparams()
+    p.Nl, p.Nppl, p.F, p.l2, p.ng = 256, 64, 1.5, 0.64, 2
+    p.xcm.x = 0.3
+    assert float(vals["MagField"]) == wfl.lib.MagField(0.25, C.byref(p))
+    assert float(vals["Exact_scalar"]) == wfl.lib.Exact(0.7, 0.9, 0)
+    assert float(vals["Exact_spinor"]) == wfl.lib.Exact(0.7, 0.9, 1)
+    assert float(vals["smTscal"]) == wfl.lib.smTscal(0.9, 0.01)
+    assert float(vals["ClassAction"]) == wfl.lib.ClassAction(1.5, 0.64)
+    if wfl.device_count() == 0:
+        assert vals["EV_without_device"] == "nan"

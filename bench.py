@@ -21,6 +21,11 @@ dominant kernel and a CPU baseline (the FP64 C oracle on a bounded sample).
       persistent launch, autotuned and replayed as a hipGraph
   c5  65536 x 8192 loops, 256 rho x 128 T, 3 x 3 images at spacing 2;
       ensemble standard error against the precision error (--scale shrinks)
+  profiles  every field profile (step ... fixflux-ff) on one 1024 x 1024
+      workload, mixed and FP64, scalar and spinor, with the mixed error
+  eaction  the whole effective action (EffAct.c:459's EAction(20, ...)) on the
+      5120 x 1024 ensemble: drop-in drivers through EV() and the batched
+      wfl_eaction, FP64 and mixed
   dropin  the reference's driver flow through EV(): one integrateT() on the
       5120 x 1024 ensemble with struct Wparams and device float4 loops, FP64
       and mixed at the reference's numerics (Simpson n = 50, its clamps)
@@ -489,6 +494,128 @@ def bench_dropin(wfl, args):
     print(json.dumps(out))
 
 
+def bench_eaction(wfl, args):
+    """The whole effective action as the reference's main computes it
+    (EffAct.c:459, EAction(20, &params)): 5120 x 1024 loops, Gaussian tube
+    F = 1, l2 = 1, ng = 40, the reference's numerics (clamps, Simpson n = 50).
+    Two ways on the same loops: the drop-in EAction driving EV() node by node
+    (EffAct.c:265-306 -> integrateRho -> integrateT -> Tfunc -> EV), and
+    wfl_eaction with every (rho, T) worldline node in one batched plan; FP64
+    and mixed for each."""
+    C = __import__("ctypes")
+    Nl, Nppl, ng, npts = 5120, 1024, 40, 20
+    fermion = 1 if args.species == "spinor" else 0
+    hip = _Hip()
+    x, y = wfl.gen_loops(1, Nl, Nppl)
+    aos = np.zeros((Nl * Nppl, 4), np.float32)
+    aos[:, 0] = x.ravel()
+    aos[:, 1] = y.ravel()
+    d_wl, d_s, d_f = hip.malloc(aos.nbytes), hip.malloc(Nl * 16), hip.malloc(Nl * 16)
+    hip.h2d(d_wl, aos)
+    hs = np.zeros((Nl, 4), np.float32)
+    hf = np.zeros((Nl, 4), np.float32)
+    p = wfl.Wparams()
+    p.Nl, p.Nppl, p.F, p.l2 = Nl, Nppl, 1.0, 1.0
+    p.Wsscal_h, p.Wsscal_d = hs.ctypes.data, d_s.value
+    p.Wsferm_h, p.Wsferm_d = hf.ctypes.data, d_f.value
+    p.worldlines = d_wl.value
+    p.nBlocks, p.nThreads, p.ng, p.fermion = (Nl + 511) // 512, 512, ng, fermion
+    geom = wfl.Geometry(kind=wfl.GAUSSIAN, F=1.0, l2=1.0)
+    wfl.set_profile(geom)
+    wfl.lib.wfl_set_verbosity(0)
+    loops = wfl.Loops(x, y)
+    out = {"workload": f"eaction_{'spinor' if fermion else 'scalar'}_{Nl}x{Nppl}_{npts}rho_40T", "n_gpus": 1,
+           "data": "synthetic (seeded Philox unit loops)"}
+    sys.stdout.flush()
+    saved = os.dup(1)
+    try:
+        os.dup2(2, 1)        # the drivers print their result lines; keep stdout for the JSON line
+        for prec, tag in ((wfl.FP64, "fp64"), (wfl.MIXED, "mixed")):
+            wfl.set_numerics(wfl.MODE_REFERENCE, prec, 50)
+            wfl.lib.wfl_ev_reset()
+            n0 = wfl.lib.wfl_ev_launches()
+            t0 = time.perf_counter()
+            r = wfl.lib.EAction(npts, C.byref(p))
+            dt = time.perf_counter() - t0
+            drop = np.ctypeslib.as_array(C.cast(r, C.POINTER(C.c_double)), shape=(ng,)).copy()
+            launches = wfl.lib.wfl_ev_launches() - n0
+            t0 = time.perf_counter()
+            groups, mean, se, ca = wfl.eaction(geom, loops, fermion, wfl.MODE_REFERENCE, prec, 50, 128, npts, 40)
+            dtb = time.perf_counter() - t0
+            out[tag] = {"dropin_seconds": dt, "dropin_ev_launches": int(launches),
+                        "dropin_action": float(drop.mean()),
+                        "dropin_action_se": float(drop.std(ddof=1) / np.sqrt(ng)),
+                        "batched_seconds": dtb, "batched_action": mean, "batched_action_se": se,
+                        "classical_action": ca,
+                        "max_group_rel_diff": float(np.max(np.abs(groups - drop)) / np.max(np.abs(drop)))}
+            log(f"eaction {tag}: drop-in {dt:.2f} s ({launches} launches), batched {dtb:.2f} s")
+    finally:
+        sys.stdout.flush()
+        os.dup2(saved, 1)
+        os.close(saved)
+    out["action_rel_diff_mixed_vs_fp64"] = abs(out["mixed"]["batched_action"] - out["fp64"]["batched_action"]) / \
+        abs(out["fp64"]["batched_action"])
+    loops.close()
+    for q in (d_wl, d_s, d_f):
+        hip.lib.hipFree(q)
+    wfl.lib.wfl_ev_reset()
+    print(json.dumps(out))
+
+
+def bench_profiles(wfl, args):
+    """Every field profile of the reference (intEV.cu:202-357, intEVff.cu) on
+    one workload: 1024 x 1024 loops, 16 rho in [0, 3] x 32 log-spaced T in
+    [1e-3, 1e2], F = 1, lambda = 1, tubedist a = 2, lambda_min = 0.2,
+    Simpson n = 2, physical mode; scalar and spinor.  Mixed and FP64 plans
+    each take their own autotuned launch; the error is the largest group-sum
+    difference over the largest group sum."""
+    Nl, N = 1024, 1024
+    x, y = wfl.gen_loops(6, Nl, N)
+    loops = wfl.Loops(x, y)
+    rhos = np.linspace(0.0, 3.0, 16)
+    Ts = np.logspace(-3, 2, 32)
+    tg, tcx, tcy, tT = tuples_grid([0], [rhos], [Ts])
+    pe = Nl * N * len(tg)
+    names = ["step", "smooth", "quadratic", "gaussian", "periodic", "spline", "fixflux", "fixflux_ff"]
+    rows = []
+    for kind, name in enumerate(names):
+        for spinor in (0, 1):
+            kw = dict(kind=kind, F=1.0, l2=1.0, a=2.0, lam_min=0.2)
+            splines = []
+            if kind == wfl.SPLINE:
+                kw["spline_table"] = 0
+                splines = [gaussian_spline_table(wfl)]
+            res = {"profile": name, "spinor": spinor}
+            sums = {}
+            for prec, tag, steps in ((wfl.MIXED, "mixed", 3), (wfl.FP64, "fp64", 2)):
+                b = wfl.Batch([wfl.Geometry(**kw)], tg, tcx, tcy, tT, mode=wfl.MODE_PHYSICAL, precision=prec, nsub=2,
+                              spinor=spinor, group=GROUP, splines=splines)
+                plan = wfl.Plan(b, loops)
+                plan.autotune()
+                plan.run()
+                wfl.check(wfl.lib.wfl_device_sync())
+                kms = [plan.run_timed() for _ in range(steps)]
+                sums[tag], _ = plan.read()
+                res[tag] = pe / (float(np.min(kms)) / 1e3)
+                res[tag + "_nonfinite"] = plan.nonfinite()
+                plan.close()
+            res["mixed_over_fp64"] = res["mixed"] / res["fp64"]
+            res["rel_err"] = float(np.max(np.abs(sums["mixed"] - sums["fp64"])) /
+                                   max(np.max(np.abs(sums["fp64"])), 1e-300))
+            # T <= 1: below the large-loop tuples whose phases (many rings of flux) reach
+            # hundreds of radians, where FP32 carries ~1e-7 |phase| into cos
+            sm = tT <= 1.0
+            res["rel_err_T_le_1"] = float(np.max(np.abs(sums["mixed"][sm] - sums["fp64"][sm])) /
+                                          max(np.max(np.abs(sums["fp64"][sm])), 1e-300))
+            log(f"profile {name} spinor {spinor}: {res['mixed']:.3e} / {res['fp64']:.3e} point-evals/s, "
+                f"err {res['rel_err']:.1e} (T <= 1: {res['rel_err_T_le_1']:.1e})")
+            rows.append(res)
+    loops.close()
+    print(json.dumps({"workload": f"profiles_{Nl}x{N}_16rho_32T", "unit": "point-evals/s", "n_gpus": 1,
+                      "point_evals_per_step": pe, "data": "synthetic (seeded Philox unit loops)",
+                      "kernel_time": "best of the timed launches (HIP events)", "profiles": rows}))
+
+
 def bench_c2(wfl, args, mode):
     cfg = config_c2()
     x, y, loops = load_loops(wfl, cfg)
@@ -564,9 +691,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--fp64-steps", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "dropin"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "dropin", "eaction", "profiles"])
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the loop ensemble (c3, c5)")
-    ap.add_argument("--species", default="both", choices=["scalar", "spinor", "both"], help="c3; dropin: spinor or scalar")
+    ap.add_argument("--species", default="both", choices=["scalar", "spinor", "both"], help="c3; dropin, eaction: spinor or scalar")
     ap.add_argument("--mode", default="physical", choices=["physical", "reference"])
     ap.add_argument("--mapping", type=int, default=-1,
                     help="0 autotune, 1 lane/tuple, 2 wave/loop, 3 lane/tuple split (default: 1 for c2, 0 otherwise)")
@@ -609,6 +736,10 @@ def main():
         return bench_c2(wfl, args, mode)
     if args.config == "dropin":
         return bench_dropin(wfl, args)
+    if args.config == "profiles":
+        return bench_profiles(wfl, args)
+    if args.config == "eaction":
+        return bench_eaction(wfl, args)
     if args.config == "c3":
         for sp in {"scalar": [0], "spinor": [1], "both": [0, 1]}[args.species]:
             run_config(wfl, args, mode, config_c3(args.scale, sp))

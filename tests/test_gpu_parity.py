@@ -272,6 +272,34 @@ def test_other_profiles_mixed(gpu, orc, loops, kind):
     assert e1 < 1e-4 and e2 < 1e-4
 
 
+@pytest.mark.parametrize("kind", ["PERIODIC", "FIXFLUX"])
+@pytest.mark.parametrize("spinor", [0, 1])
+def test_other_profiles_mixed_small_loops_at_centre(gpu, orc, loops, kind, spinor):
+    """Small loops at and near a tube centre, where f(rho^2) ~ rho^2: the
+    reference's forms of the periodic and fixflux profiles are 1 minus nearly
+    1 there (relative error ~1e-7 / f in FP32, ~1e-3 in the phase at
+    T = 1e-3); the FP32 kernels use equivalent forms without the
+    cancellation and keep the per-loop phases within 2e-6 (for the fixflux
+    form, the reference's phi(0) = 1 - q2*/q2 of the rounded q2 included).
+    The periodic rings sit at a = 10 lambda: nearer rings leave f(0) > 0 (the
+    sum of their tails), a point flux whose pure-gauge part small loops
+    beside the axis cancel in FP32 like a distant tube's.  Loops further out
+    (|x_cm|/sqrt(T) ~ 10) carry the symmetric gauge's ~1e-7 |x_cm|/sqrt(T)."""
+    k = getattr(gpu, kind)
+    a = 8.0 if kind == "PERIODIC" else 2.0
+    geoms = [gpu.Geometry(kind=k, F=0.5, l2=0.64, a=a, lam_min=0.2)]
+    og = [orc.Geom(getattr(orc, kind), F=0.5, l2=0.64, tubedist=a, lmin=0.2)]
+    gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, 1, 2, spinor, gpu.MIXED, 1,
+                                tuples([0.0, 0.05, 0.15], [1e-3, 1e-2]))
+    errs = []
+    for t in range(pl.shape[0]):
+        e_phase = relerr(pl[t, :, :3], rpl[t, :, :3])
+        e_spin = relerr(pl[t, :, 3:], rpl[t, :, 3:]) if spinor else 0.0
+        print(f"  {kind} tuple {t}: phase {e_phase:.2e} spin {e_spin:.2e}")
+        errs.append(max(e_phase, e_spin))
+    assert max(errs) < 2e-6, errs
+
+
 @pytest.mark.parametrize("spinor", [0, 1])
 def test_gaussian_mixed_small_loops_in_core(gpu, orc, loops, spinor):
     """Loops far smaller than the tube and inside its core (lambda = 2,

@@ -528,6 +528,8 @@ static void fill_geom(const wfl_geometry& g, const SplineBlob* sb, bool scaled, 
   }
   d.c8f = (float)(1.0e-8 * sg * sg);
   d.c10f = (float)(1.0e-10 * sg * sg);
+  // fixflux phi(0) = 1 - q2*/q2, q2* = (e^-1 - E1(1))/2 (E1(1) = 0.219383934395520273677...)
+  d.fix_c0f = (float)(1.0 - 0.5 * (0.36787944117144232160 - 0.21938393439552027368) / d.q2);
 }
 
 static void fill_tuple(double sg, const DevGeom& dg, int gi, double cx, double cy, double rtT, int nloops,

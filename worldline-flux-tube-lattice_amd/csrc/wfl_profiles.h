@@ -44,6 +44,7 @@ struct DevGeom {
   float c8f, c10f;                                  // reference clamps 1e-8, 1e-10 in FP32 kernel units
   int acc_on;                                       // FP32 Gaussian: accurate node where loops reach a core
   float acc_rmax;                                   // largest loop radius of the ensemble (unit loops)
+  float fix_c0f;                                    // fixflux: phi(0) of the reference's form, 1 - q2*/q2
 };
 
 // One evaluation tuple; the image-window centre (m0, n0) is computed on the
@@ -82,8 +83,9 @@ template <> struct Math<double> {
 
 template <class R> struct GP;   // geometry parameters in the kernel's type
 template <> struct GP<float> {
-  float F, l2, inv_l2, a, lmin, q1, q2;
-  __device__ GP(const DevGeom& g) : F(g.Ff), l2(g.l2f), inv_l2(g.inv_l2f), a(g.af), lmin(g.lminf), q1(g.q1f), q2(g.q2f) {}
+  float F, l2, inv_l2, a, lmin, q1, q2, c0;
+  __device__ GP(const DevGeom& g)
+      : F(g.Ff), l2(g.l2f), inv_l2(g.inv_l2f), a(g.af), lmin(g.lminf), q1(g.q1f), q2(g.q2f), c0(g.fix_c0f) {}
 };
 template <> struct GP<double> {
   double F, l2, inv_l2, a, lmin, q1, q2;
@@ -137,7 +139,7 @@ __device__ __forceinline__ double gauss_g(double u, double inv_l2, double& e) {
 // ------------------------------------------------ remaining profiles
 template <class R>
 __device__ R expint_neg(R x) {   // Ei(x) = -E1(-x), x < -1 (intEV.cu:29-61), converged to R
-  const R EPS = sizeof(R) == 4 ? (R)1.0e-6 : (R)1.0e-15;
+  const R EPS = sizeof(R) == 4 ? (R)2.0e-7 : (R)1.0e-15;
   R b = -x + (R)1, c = (R)1e30, d = Math<R>::div((R)1, b), h = d;
   for (int i = 1; i <= 1000; ++i) {
     R a = -(R)i * (R)i;
@@ -155,8 +157,32 @@ template <class R> __device__ R bump(R x) {
   return (x * x < (R)0.999) ? Math<R>::exp(Math<R>::div((R)-1, (R)1 - x * x)) : (R)0;
 }
 
+// The fixflux core profile, phi(x) = 1 - (om e^{-1/om} + Ei(-1/om)) / (2 q2)
+// with om = 1 - x^2, is phi(0) plus (1/(2 q2)) times the integral of
+// e^{-1/t} over [1 - x^2, 1] (d/dom of om e^{-1/om} + Ei(-1/om) is e^{-1/om}).
+// phi(0) = 1 - q2*/q2 with q2* = (e^{-1} - E1(1))/2, the exact value of the
+// constant: zero up to the rounding of the q2 the profile is given.  Near
+// the axis the first form is 1 minus nearly 1; the FP32 kernels use the
+// integral there, by 6-point Gauss-Legendre (error below 1e-9 for
+// x^2 <= 1/2: the integrand's singularity at t = 0 lies 3 half-widths from
+// the centre of the interval).
+__device__ __forceinline__ float phi_fix_core_f32(float q2, float c0, float x) {
+  const float gx[3] = {0.2386191861f, 0.6612093865f, 0.9324695142f};
+  const float gw[3] = {0.4679139346f, 0.3607615730f, 0.1713244924f};
+  const float h = 0.5f * x * x, c = 1.0f - h;   // [1 - x^2, 1] = c -+ h
+  float sum = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float t1 = fmaf(-h, gx[k], c), t2 = fmaf(h, gx[k], c);
+    sum = fmaf(gw[k], Math<float>::exp(-1.0f / t1) + Math<float>::exp(-1.0f / t2), sum);
+  }
+  return fmaf(sum, h / (2.0f * q2), c0);
+}
+
 template <class R> __device__ R phi_fix(const GP<R>& g, R x, bool ff) {
   R om = (R)1 - x * x;
+  if constexpr (sizeof(R) == 4)
+    if (x * x <= 0.5f) return phi_fix_core_f32(g.q2, g.c0, x);
   if (x < (ff ? (R)1 : (R)0.999))
     return (R)1 - Math<R>::div((R)0.5, g.q2) * (om * Math<R>::exp(Math<R>::div((R)-1, om)) + expint_neg<R>(Math<R>::div((R)-1, om)));
   return (R)1;
@@ -181,10 +207,35 @@ template <class R> __device__ R chi_fix(const GP<R>& g, R x, R n, R lam, bool ff
   return Math<R>::div((R)2 * n * g.a * g.q1, lam);
 }
 
+// With the i = 0 ring in the window, the reference's sum
+//   a/(lam ln2) * [-(rho/a) e/(1+e) - (lam/a) log(1+e)] + 1,  e = exp(-t), t = rho/lam,
+// is 1 minus nearly 1 near the axis (f ~ t^2/(8 ln2)).  It equals
+// [(t/2) tanh(t/2) - log cosh(t/2)] / ln2 (both vanish at t = 0 and have the
+// derivative (t/4) sech^2(t/2) / ln2), whose two terms differ by a factor
+// of about 2 for t < 2; the FP32 kernels use that form there and the sum
+// itself (1 minus at most 0.53) beyond.
+__device__ __forceinline__ float peri_core_f32(float t) {
+  if (t >= 2.0f) {
+    const float e = Math<float>::exp(-t);
+    return 1.0f - (t * e / (1.0f + e) + log1pf(e)) * 1.44269504f;
+  }
+  // with m = e^y - 1: tanh y = m(m + 2)/(m(m + 2) + 2), cosh y - 1 = m^2/(2(1 + m))
+  const float y = 0.5f * t, m = expm1f(y), mm = m * (m + 2.0f);
+  const float lc = log1pf(m * m / (2.0f * (1.0f + m)));
+  return (y * (mm / (mm + 2.0f)) - lc) * 1.44269504f;
+}
+
 template <class R> __device__ R flperi(const GP<R>& g, R rho2) {   // intEV.cu:202-232
   R rho = Math<R>::sqrt(rho2), lam = Math<R>::sqrt(g.l2);
   R flraf = Math<R>::floor(Math<R>::div(rho, g.a)), f = (R)0;
   int fl = (int)flraf;
+  if constexpr (sizeof(R) == 4) {
+    if (fl <= 2) {              // the i = 0 ring is in the window: stable core form plus the rings i > 0
+      for (int i = 1; i <= fl + 2; ++i)
+        f += Math<R>::div((R)i, (R)1 + Math<R>::exp(-Math<R>::div(rho - (R)i * g.a, lam)));
+      return Math<R>::div(g.a, lam * (R)0.69314718055994530942) * f + peri_core_f32(Math<R>::div(rho, lam));
+    }
+  }
   if (fl - 2 > 0) f = (R)0.5 * (flraf - (R)3) * (flraf - (R)2);
   for (int i = fl - 2; i <= fl + 2; ++i) {
     if (i == 0) {

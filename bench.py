@@ -196,9 +196,10 @@ def accuracy(wfl, cfg, gs_mixed, gs_64):
     the action (the sum over x_cm), the largest error in units of the
     standard error of the mean over loop groups among the x_cm where
     |L_eff| >= 1e-3 of its largest value (far outside a tube both L_eff and
-    its SEM vanish), and the median SEM/|L_eff| (above ~0.1 the cell is
+    its SEM vanish), and the median SEM/|L_eff| (near 1 the cell is
     dominated by Monte-Carlo noise: the sign problem of cos(Phi) cosh(S) at
-    strong fields)."""
+    strong fields; action_rel_err_max_well_conditioned takes the geometries
+    below 0.5)."""
     res = {}
     for policy, tag in ((1, "tfunc_policy"), (0, "worldline_all_T")):
         Lm = leff_table(wfl, cfg, gs_mixed, policy)
@@ -221,8 +222,10 @@ def accuracy(wfl, cfg, gs_mixed, gs_64):
     w_m = 1.0 + gs_mixed.sum(axis=1) @ c / cfg["Nl"]
     w_d = 1.0 + gs_64.sum(axis=1) @ c / cfg["Nl"]
     res["W_max_rel_err"] = float(np.max(np.abs(w_m - w_d) / np.abs(w_d)))
+    # geometries whose ensemble estimate is not dominated by noise (the sign problem of strong
+    # fields drives the median SEM/|L_eff| to ~1)
     well = [p["action_rel_err"] for p in res["tfunc_policy"]
-            if p["median_sem_over_leff"] is not None and p["median_sem_over_leff"] < 0.1]
+            if p["median_sem_over_leff"] is not None and p["median_sem_over_leff"] < 0.5]
     res["action_rel_err_max_well_conditioned"] = float(max(well)) if well else None
     res["max_err_over_sem_all"] = float(max((p["max_err_over_sem"] or 0.0) for p in res["tfunc_policy"]))
     return res

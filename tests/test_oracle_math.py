@@ -255,3 +255,37 @@ def test_lattice_field_window(orc):
     avg = np.mean([orc.field_at(lat, x, y)[0] for x in c for y in c])
     e = 0.30282212
     assert avg == pytest.approx(2 * np.pi * 1.0 / e / 4.0, rel=1e-6)
+
+
+def test_fp32_profile_identities():
+    """The identities behind the FP32 kernels' forms of two profiles, checked
+    against the reference's expressions (intEV.cu:162-174, 202-232) in
+    50-digit arithmetic:
+    * the periodic sum's central ring plus 1 equals
+      [(t/2) tanh(t/2) - log cosh(t/2)] / ln 2 (t = rho/lambda), with
+      tanh y = m(m+2)/(m(m+2)+2) and cosh y - 1 = m^2/(2(1+m)), m = e^y - 1;
+    * the fixflux phi(x) equals phi(0) + (1/(2 q2)) int_{1-x^2}^1 e^{-1/t} dt,
+      phi(0) = 1 - q2*/q2, q2* = (e^-1 - E1(1))/2."""
+    import mpmath as mp
+    mp.mp.dps = 50
+    for t in [1e-3, 0.05, 0.4, 1.0, 1.9]:
+        t = mp.mpf(t)
+        e = mp.exp(-t)
+        ref = 1 - (t * e / (1 + e) + mp.log(1 + e)) / mp.log(2)
+        y = t / 2
+        m = mp.expm1(y)
+        th = m * (m + 2) / (m * (m + 2) + 2)
+        lc = mp.log(1 + m * m / (2 * (1 + m)))
+        assert abs((y * th - lc) / mp.log(2) - ref) <= 1e-30 * abs(ref)
+        assert abs(th - mp.tanh(y)) < 1e-28 and abs(lc - mp.log(mp.cosh(y))) < 1e-28
+    q2_star = (mp.exp(-1) - mp.e1(1)) / 2
+    assert abs(q2_star - mp.mpf("0.0742477533879612")) < 1e-15
+    for q2 in (float(q2_star), 0.0742478, float(np.float32(0.0742477533879612))):
+        q2 = mp.mpf(q2)
+        phi0 = 1 - q2_star / q2
+        for x in [1e-3, 0.1, 0.5, 0.7]:
+            x = mp.mpf(x)
+            om = 1 - x * x
+            ref = 1 - (om * mp.exp(-1 / om) + mp.ei(-1 / om)) / (2 * q2)
+            integral = mp.quad(lambda s: mp.exp(-1 / s), [om, 1])
+            assert abs(phi0 + integral / (2 * q2) - ref) < 1e-25

@@ -755,7 +755,7 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
   p->b.geoms = nullptr; p->b.splines = nullptr; p->b.tup_geom = nullptr;
   p->b.tup_cx = p->b.tup_cy = p->b.tup_T = nullptr; p->b.geom_nloops = nullptr;
   p->L = L;
-  const int Nl = L->Nl, N = L->N;
+  const int Nl = L->Nl;
   p->ngroups = (Nl + b->group - 1) / b->group;
 
   // splines: one blob of every table
@@ -830,12 +830,11 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
   p->lg_images = lg_images;
   p->ntasks = p->ntasks1 = (int)plan_task_list(p, 1).size();
   // mapping (autotune heuristic: lane-per-tuple needs enough lane groups)
+  // (mapping 3 measured no faster than 1 on the bench workload: it is opt-in or autotuned only)
   int mapping = b->mapping;
-  const double perloop_bytes = 48.0 * (double)std::max(b->ntup, 1) * (double)Nl;
   if (mapping == 0) {
     double util = b->ntup > 0 ? (double)b->ntup / (64.0 * std::max(nlg, 1)) : 0.0;
     mapping = (util >= 0.5 && p->ntasks >= 512) ? 1 : 2;
-    (void)perloop_bytes;   // mapping 3 measured no faster than 1 on the bench workload: opt-in only
   }
   if (mapping == 3 && !p->prof_gauss) mapping = 1;
   p->mapping = mapping;
@@ -880,7 +879,6 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
     hipError_t e = hipMemset(p->d_nonfinite, 0, sizeof(unsigned long long));
     if (e != hipSuccess) { wfl_set_error("hipMemset: %s", hipGetErrorString(e)); wfl_plan_destroy(p); return WFL_ERR_DEVICE; }
   }
-  (void)N;
   *out = p;
   return WFL_OK;
 }

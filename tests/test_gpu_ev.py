@@ -117,6 +117,49 @@ def test_ev_mixed_precision(setup, orc):
     wfl.set_numerics(wfl.MODE_REFERENCE, wfl.FP64, 50)
 
 
+@pytest.mark.parametrize("mode,prec,nsub,tol", [(0, 0, 50, 3e-7), (1, 1, 2, 2e-5)])
+def test_ev_spline_profile(setup, orc, mode, prec, nsub, tol):
+    """profile == spline through EV(): the caller's device tables
+    (params.flcoefs, params.rho2sp, Nspline knots as periodic.c:129-151 lays
+    them out), in the reference's numerics (FP64, n = 50; exact interval
+    search) and in mixed physical mode (the direct-index node on these
+    uniform knots), against the oracle's bisection."""
+    wfl, hip, x, y, d_wl, d_s, d_f, hs, hf = setup
+    n = 512
+    rho = (np.arange(n) + 1) * (20.0 / n)
+    knots = (rho * rho).astype(np.float32)
+    coefs = wfl.spline_build(-np.expm1(-knots.astype(np.float64)), knots)
+    d_kn, d_cf = hip.malloc(knots.nbytes), hip.malloc(coefs.nbytes)
+    hip.h2d(d_kn, knots)
+    hip.h2d(d_cf, np.ascontiguousarray(coefs, np.float32))
+    wfl.set_profile(wfl.Geometry(kind=wfl.SPLINE, F=1.0, l2=1.0, a=1.0, spline_table=0), n)
+    wfl.set_numerics(mode, prec, nsub)
+    wfl.lib.wfl_ev_reset()
+    WL = np.arange(128, dtype=np.int32)
+    g = orc.Geom(orc.SPLINE, F=1.0, l2=1.0, tubedist=1.0, knots=knots, coefs=coefs, quantize="float")
+    try:
+        for fermion, rho_cm, T in ((0, 0.4, 0.3), (1, 1.1, 0.05)):
+            p = make_params(wfl, d_wl, d_s, d_f, hs, hf, 1.0, 1.0, rho_cm, fermion)
+            p.flcoefs, p.rho2sp = d_cf.value, d_kn.value
+            print(f"EV spline mode={mode} prec={prec} fermion={fermion}", flush=True)
+            ev = wfl.lib.EV(T, C.byref(p), WL.ctypes.data)
+            assert np.isfinite(ev)
+            rt = float(np.float32(np.sqrt(T)))
+            for l in range(0, NL, 7):
+                sc, fe = orc.wilson_loop(g, mode, nsub, fermion, x[l], y[l], float(np.float32(rho_cm)), 0.0, rt)
+                scale = max(np.max(np.abs(sc)), 1e-300)
+                assert np.max(np.abs(hs[l, :3] - sc)) <= tol * scale, (l, hs[l, :3], sc)
+                if fermion:
+                    scf = max(np.max(np.abs(fe)), 1e-300)
+                    assert np.max(np.abs(hf[l, :3] - fe)) <= tol * scf, (l, hf[l, :3], fe)
+    finally:
+        hip.free(d_kn)
+        hip.free(d_cf)
+        wfl.set_profile(wfl.Geometry(kind=wfl.GAUSSIAN))
+        wfl.set_numerics(wfl.MODE_REFERENCE, wfl.FP64, 50)
+        wfl.lib.wfl_ev_reset()
+
+
 def test_integrateT_matches_oracle(setup, orc):
     """integrateT (Tint.c:100-175) through Tfunc and EV against the oracle's
     restatement of the same t-grid Simpson rule and branch policy."""

@@ -160,6 +160,34 @@ def test_ev_spline_profile(setup, orc, mode, prec, nsub, tol):
         wfl.lib.wfl_ev_reset()
 
 
+@pytest.mark.parametrize("kind", ["STEP", "SMOOTH", "QUADRATIC", "PERIODIC", "FIXFLUX", "FIXFLUX_FF"])
+def test_ev_other_profiles(setup, orc, kind):
+    """The remaining profiles through EV() (the intT.h profile switch,
+    tubedist a = 2, lmin = 0.2) in the reference's numerics, FP64, against
+    the oracle: per-loop float4 outputs to their float rounding."""
+    wfl, hip, x, y, d_wl, d_s, d_f, hs, hf = setup
+    wfl.set_profile(wfl.Geometry(kind=getattr(wfl, kind), F=0.8, l2=0.64, a=2.0, lam_min=0.2))
+    wfl.set_numerics(wfl.MODE_REFERENCE, wfl.FP64, 50)
+    wfl.lib.wfl_ev_reset()
+    WL = np.arange(128, dtype=np.int32)
+    g = orc.Geom(getattr(orc, kind), F=0.8, l2=0.64, tubedist=2.0, lmin=0.2, quantize="float")
+    try:
+        for fermion, rho_cm, T in ((0, 0.3, 0.2), (1, 1.4, 0.6)):
+            p = make_params(wfl, d_wl, d_s, d_f, hs, hf, 0.8, 0.64, rho_cm, fermion)
+            print(f"EV {kind} fermion={fermion}", flush=True)
+            ev = wfl.lib.EV(T, C.byref(p), WL.ctypes.data)
+            assert np.isfinite(ev)
+            rt = float(np.float32(np.sqrt(T)))
+            for l in range(0, NL, 9):
+                sc, fe = orc.wilson_loop(g, 0, 50, fermion, x[l], y[l], float(np.float32(rho_cm)), 0.0, rt)
+                assert np.max(np.abs(hs[l, :3] - sc)) <= 3e-7 * max(np.max(np.abs(sc)), 1e-300), (l, hs[l, :3], sc)
+                if fermion:
+                    assert np.max(np.abs(hf[l, :3] - fe)) <= 3e-7 * max(np.max(np.abs(fe)), 1e-300), (l, hf[l], fe)
+    finally:
+        wfl.set_profile(wfl.Geometry(kind=wfl.GAUSSIAN))
+        wfl.lib.wfl_ev_reset()
+
+
 def test_integrateT_matches_oracle(setup, orc):
     """integrateT (Tint.c:100-175) through Tfunc and EV against the oracle's
     restatement of the same t-grid Simpson rule and branch policy."""

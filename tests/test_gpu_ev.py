@@ -188,6 +188,48 @@ def test_ev_other_profiles(setup, orc, kind):
         wfl.lib.wfl_ev_reset()
 
 
+def test_rho_drivers_compose(setup):
+    """The rho-integral drivers of EffAct.c:15-135 built on integrateT:
+    rhogrand(rho) = rho * integrateT(40) at x_cm = (rho, 0); transformRho(t)
+    = rhogrand((1 - t)/t) / t^2; integrateRhotoa(n) = h/3 [rg(0) + rg(a/2)
+    + sum_k w_k rg(k h)], h = a/(2n), with the reference's weights (2 for
+    odd k, 4 for even k, EffAct.c:113-126)."""
+    wfl, hip, x, y, d_wl, d_s, d_f, hs, hf = setup
+    ng = 2
+
+    def arr(ptr):
+        return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_double)), shape=(ng,)).copy()
+
+    a = 1.5
+    wfl.set_profile(wfl.Geometry(kind=wfl.GAUSSIAN, F=1.0, l2=1.0, a=a))
+    wfl.set_numerics(wfl.MODE_REFERENCE, wfl.MIXED, 50)
+    wfl.lib.wfl_ev_reset()
+    wfl.lib.wfl_set_verbosity(0)
+    try:
+        p = make_params(wfl, d_wl, d_s, d_f, hs, hf, 1.0, 1.0, 0.0, 0, ng=ng)
+        print("rho drivers", flush=True)
+
+        def rg(rho):
+            return arr(wfl.lib.rhogrand(rho, C.byref(p)))
+
+        r = rg(0.3)
+        p.xcm.x = 0.3
+        assert np.array_equal(r, 0.3 * arr(wfl.lib.integrateT(40, C.byref(p))))
+        assert np.allclose(arr(wfl.lib.transformRho(0.4, C.byref(p))), rg(1.5) / 0.16, rtol=1e-14)
+        n = 3
+        h = a / (2 * n)
+        want = rg(0.0) + rg(a / 2)
+        for k in range(1, n + 1):
+            want = want + (2.0 if k % 2 else 4.0) * rg(h * k)
+        got = arr(wfl.lib.integrateRhotoa(n, C.byref(p)))
+        assert np.allclose(got, h / 3 * want, rtol=1e-13), (got, h / 3 * want)
+    finally:
+        wfl.lib.wfl_set_verbosity(1)
+        wfl.set_profile(wfl.Geometry(kind=wfl.GAUSSIAN))
+        wfl.set_numerics(wfl.MODE_REFERENCE, wfl.FP64, 50)
+        wfl.lib.wfl_ev_reset()
+
+
 def test_integrateT_matches_oracle(setup, orc):
     """integrateT (Tint.c:100-175) through Tfunc and EV against the oracle's
     restatement of the same t-grid Simpson rule and branch policy."""

@@ -88,8 +88,6 @@ __global__ void k_records(const float* __restrict__ x, const float* __restrict__
   r[3] = (R)(4.0 * ((double)yl[s] - wy));
   r[4] = (R)((double)xl[s] - (double)xl[p]);
   r[5] = (R)((double)yl[s] - (double)yl[p]);
-  r[6] = (R)0;
-  r[7] = (R)0;
 }
 
 // Largest |w|^2 over all points: block maximum, then one atomic per block

@@ -4,7 +4,7 @@
 // Data: each loop point i carries a record of six values
 //   (w_x, w_y, 4 dw_x, 4 dw_y, e_x, e_y),  dw = w_succ(i) - w_i,  e = w_succ(i) - w_pred(i)
 // where succ/pred follow the sub-loop map of getzp1 (intEV.cu:462-506).
-// Records are stored point-major inside each loop ([Nl][N][REC], REC = 8: six values and two pad slots, 32-byte aligned) so that one
+// Records are stored point-major inside each loop ([Nl][N][REC], REC = 6 values, 24 B in FP32) so that one
 // wavefront walking one loop reads them with scalar loads.
 //
 // Two mappings of work to the machine:
@@ -32,8 +32,8 @@ namespace wfl {
 // one of its chunks when groups are split into several tasks).
 struct Task { int lanegroup, loop0, nloop, slot; };
 
-// values per point record (w, dw, e and two pad slots: 32 B in FP32)
-constexpr int REC = 8;
+// values per point record (w, 4 dw, e: 24 B in FP32)
+constexpr int REC = 6;
 
 __device__ __forceinline__ int succ_idx(int i, int N) {
   if (i & 1) return (i == N - 1) ? 1 : i + 2;
@@ -332,7 +332,7 @@ __device__ __forceinline__ void tuple_task(
 
   double G[3] = {0.0, 0.0, 0.0};
   for (int l = loop0; l < loop0 + nloop; ++l) {
-    const R* __restrict__ rl = (const R*)__builtin_assume_aligned(rec + (size_t)l * N * REC, REC * sizeof(R));
+    const R* __restrict__ rl = (const R*)__builtin_assume_aligned(rec + (size_t)l * N * REC, 2 * sizeof(R));
     double P[3] = {0.0, 0.0, 0.0}, S[3] = {0.0, 0.0, 0.0};
     loop_sums<R, PROF, SPINOR, MODE, NSUB>(rl, N, M, lattice, tp, gm, cx, cy, lat, lc, kind, gp, sp, nsub_rt, P, S);
     double sc[3], fe[3];

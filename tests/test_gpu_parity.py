@@ -217,6 +217,29 @@ def test_spline_uniform_grid_at_knots(gpu, orc, loops, spinor):
     assert np.all(e_uni < 2.0 * e_exact + 1e-6), (e_uni, e_exact)
 
 
+@pytest.mark.parametrize("spinor", [0, 1])
+def test_spline_small_tables(gpu, orc, loops, spinor):
+    """The smallest tables the spline set-up builds (3 and 5 knots), a
+    uniform table whose first knot is rho = 0, and one whose first knot lies
+    beyond one spacing (r0 > h: not the direct-index layout, so the exact
+    search): mixed physical mode, against the oracle."""
+    cases = [((np.arange(3) + 1) * 0.7, 1), ((np.arange(5) + 1) * 0.5, 1), (np.arange(9) * 0.4, 1),
+             (np.arange(8) * 0.3 + 0.9, 0)]
+    for rho_k, want_uni in cases:
+        knots = (rho_k * rho_k).astype(np.float32)
+        coefs = gpu.spline_build(-np.expm1(-knots.astype(np.float64)), knots)
+        geoms = [gpu.Geometry(kind=gpu.SPLINE, F=1.0, l2=1.0, a=1.0, spline_table=0)]
+        og = [orc.Geom(orc.SPLINE, F=1.0, l2=1.0, tubedist=1.0, knots=knots, coefs=coefs)]
+        seen = []
+        gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, 1, 2, spinor, 1, 1,
+                                    tuples([0.0, 0.6, 2.5], [0.01, 0.3, 3.0]), splines=[(knots, coefs)],
+                                    on_plan=lambda p: seen.append(p.uniform_splines))
+        e1, e2 = relerr(pl, rpl), relerr(gs, rgs)
+        print(f"  {len(knots)} knots from rho {rho_k[0]:.1f}: uniform {seen[0]} perloop {e1:.2e} gsum {e2:.2e}")
+        assert seen == [want_uni]
+        assert e1 < 2e-5 and e2 < 1e-5
+
+
 @pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 5e-5)])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_spline_irregular_knots_and_mixed_profiles(gpu, orc, loops, prec, tol, mode):

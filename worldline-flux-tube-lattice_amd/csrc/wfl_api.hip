@@ -345,7 +345,7 @@ static double unit_scale(const wfl_geometry& g, bool scaled) {
 // u is at most s + 2 for every bucket; nb is the smallest of n/2, n, 2n, 4n,
 // 8n (at most 16384) for which it is, else 2n with the scan.
 // Uniform-grid rows (FP32 physical mode, tables whose knots lie on
-// r_j = r0 + j h with r0 <= h), at the front of the blob: the n + 1
+// r_j = r0 + j h), at the front of the blob: the n + 1
 // coefficient rows, then their n + 1 left knots as floats (0, k_0, ...,
 // k_{n-2}, k_{n-1}) padded to whole float4.  uni_f4 is the size of that
 // prefix, the part the uniform-grid kernels stage in LDS.
@@ -362,14 +362,14 @@ struct SplineBlob {
   bool all_uni = false;  // every table has them
 };
 
-// Knots on a uniform grid in r = sqrt(rho^2) that starts at or below one
-// spacing: with inv_h = fl32(1/h) and off = fl32(1 - r0 inv_h), every knot
-// k_j has sqrt(k_j) inv_h + off within 1e-4 of j + 1.
+// Knots on a uniform grid in r = sqrt(rho^2): with inv_h = fl32(1/h) and
+// off = fl32(1 - r0 inv_h), every knot k_j has sqrt(k_j) inv_h + off within
+// 1e-4 of j + 1.
 static bool spline_uniform(const float* k, int n, SplineTab& st) {
   if (n < 3 || !(k[0] >= 0.0f)) return false;
   const double r0 = sqrt((double)k[0]), r1 = sqrt((double)k[n - 1]);
   const double h = (r1 - r0) / (double)(n - 1);
-  if (!(h > 0.0) || r0 > h * (1.0 + 1e-4)) return false;
+  if (!(h > 0.0)) return false;
   const float inv_h = (float)(1.0 / h);
   const float off = (float)(1.0 - r0 * (double)inv_h);
   for (int j = 0; j < n; ++j) {

@@ -335,10 +335,10 @@ __device__ __forceinline__ SplineRef spline_ref(const float4* blob, const DevGeo
 }
 
 // Uniform-grid spline, FP32 physical mode: g(u) = f(u)/u and f'(u) from one
-// row in LDS, with no search.  The knots lie on r_j = r0 + j h with
-// r0 <= h (checked on the host to 1e-4 h), so row i = trunc((sqrt(u) - r0)/h) + 1,
-// at most n, is interval i - 1; row 0 is the line s u below the first knot
-// and row n the constant past the last (1 - r0/h > -1, so no lower clamp).
+// row in LDS, with no search.  The knots lie on r_j = r0 + j h (checked on
+// the host to 1e-4 h), so row i = trunc((sqrt(u) - r0)/h) + 1, clamped to
+// [0, n], is interval i - 1; row 0 is the line s u below the first knot and
+// row n the constant past the last.
 // Row i holds the coefficients (c0, c1, c2, c3) of the exact lookup's row i
 // (16 B, dense) and, in an array of its own, the row's left knot (0 for
 // row 0, k_{n-1} for row n).  Near a knot, rounding of sqrt(u) may pick
@@ -347,7 +347,7 @@ __device__ __forceinline__ SplineRef spline_ref(const float4* blob, const DevGeo
 // the FP32 rounding level.  u > 0 always holds here (the FP32 kernels add
 // 1e-30).
 __device__ __forceinline__ float spline_u_g(float u, const SplineRef& sp, int n, bool need_fp, float& fp) {
-  const float t = fminf(fmaf(__builtin_amdgcn_sqrtf(u), sp.uinv_h, sp.uoff), (float)n);
+  const float t = __builtin_amdgcn_fmed3f(fmaf(__builtin_amdgcn_sqrtf(u), sp.uinv_h, sp.uoff), 0.0f, (float)n);
   const int i = (int)t;
   const float4 c = sp.ucf[i];
   const float dx = u - sp.ukn[i];

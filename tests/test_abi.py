@@ -254,3 +254,23 @@ def test_c_driver_links_and_agrees(wfl, tmp_path):
     assert float(vals["ClassAction"]) == wfl.lib.ClassAction(1.5, 0.64)
     if wfl.device_count() == 0:
         assert vals["EV_without_device"] == "nan"
+
+
+def test_evmean_matches_oracle(wfl, orc):
+    """EVMean (Integrand.c:325-407), host code: the library's against the
+    oracle's on random float4 phases and spin sums, scalar and spinor,
+    including values past the reference's |.| > 600 cosh guard."""
+    rng = np.random.default_rng(5)
+    ws = rng.normal(scale=0.8, size=(300, 4)).astype(np.float32)
+    wf = rng.normal(scale=0.5, size=(300, 4)).astype(np.float32)
+    wf[7, 0] = 650.0            # past the guard
+    WL = rng.permutation(300)[:128].astype(np.int32)
+    for fermion in (0, 1):
+        out = C.c_double(0.0)
+        wfl.lib.EVMean(C.addressof(out), ws.ctypes.data, wf.ctypes.data, 128, WL.ctypes.data, 0.7, fermion)
+        want = orc.evmean(ws, wf, WL, fermion)
+        assert out.value == pytest.approx(want, rel=1e-13, abs=1e-300)
+    WL[0] = 7                   # the guarded loop in the group
+    out = C.c_double(0.0)
+    wfl.lib.EVMean(C.addressof(out), ws.ctypes.data, wf.ctypes.data, 128, WL.ctypes.data, 0.7, 1)
+    assert out.value == pytest.approx(orc.evmean(ws, wf, WL, 1), rel=1e-13)

@@ -43,6 +43,8 @@ for _n, _r, _a in [
     ("orc_gen_loops", None, [C.c_uint32, _I, _I, _P, _P]),
     ("orc_succ", _I, [_I, _I]),
     ("orc_f", _D, [_P, _D]),
+    ("orc_f_in_mode", _D, [_P, _I, _D]),
+    ("orc_spline_low_slope", _D, [_P, _P]),
     ("orc_fp", _D, [_P, _D]),
     ("orc_expint", _D, [_D]),
     ("orc_interp", _D, [_D, _P, _P, _I]),

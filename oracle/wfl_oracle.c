@@ -4,22 +4,32 @@
  *
  * Pins (each function is checked by a test that a plausible mistake fails):
  *   orc_philox4x32_10   Random123 known-answer vectors            tests/test_oracle_loops.py
- *   orc_gen_loops       closure, CM = 0, increment variance, Levy-area law
- *                       eBT/sinh(eBT), committed SHA-256 of the FP32 buffer  test_oracle_loops.py
- *   succ (getzp1)       brute-force orbit enumeration N = 4, 8, 16   test_oracle_wilson.py
- *   orc_wilson_loop     shoelace identity Phi = (2F/l2) Area (step profile inside
- *                       the tube); Simpson exact for cubics; arctan closed form for
- *                       the smooth profile with O(h^4) convergence; hand-built
- *                       B<0 segments for the reference clamps; rotation,
- *                       reflection and reversal invariances; lattice K=1 limit,
- *                       lattice-vector translation invariance       test_oracle_wilson.py
- *   orc_f / orc_fp      closed forms, derivative by finite differences, expint
- *                       against scipy.special.exp1                 test_oracle_profiles.py
- *   orc_wminus1 /       hand-computed vectors, Richardson coefficients cancel
- *   orc_evmean          1/N and 1/N^2 terms, cosh guard            test_oracle_evmean.py
- *   orc_exact, smT*     Taylor series of Exact at small T (mpmath)   test_oracle_tfunc.py
- *   orc_spline          scipy CubicSpline(bc_type=clamped, 0 slopes)  test_oracle_spline.py
- *   orc_logsimpson      exact for polynomials in ln T of degree <= 3  test_oracle_tfunc.py
+ *   orc_gen_loops       numpy restatement, closure, CM = 0, increment variance,
+ *                       Levy-area law eBT/sinh(eBT) (through orc_eval_batch),
+ *                       committed SHA-256 of the FP32 buffers      test_oracle_loops.py
+ *   orc_succ (getzp1)   brute-force orbit enumeration N = 4 ... 64   test_oracle_wilson.py
+ *   orc_wilson_loop     shoelace identity Phi = (2F/l2) Area (constant field, both
+ *                       modes); Simpson exact for quadratic integrands; arctan
+ *                       closed form for the smooth profile with O(h^4) convergence;
+ *                       hand-built B<0 segments for the reference clamps; rotation,
+ *                       reflection and reversal invariances; sqrt(T) w scaling;
+ *                       lattice K=1 limit, far images, lattice-vector translation
+ *                                                                  test_oracle_wilson.py
+ *   orc_eval_batch      group sums of orc_wilson_loop (Levy-area law)  test_oracle_loops.py
+ *   orc_f / orc_fp      closed forms, f' by finite differences, periodic rings   test_oracle_math.py
+ *   orc_expint          scipy.special.expi                          test_oracle_math.py
+ *   orc_spline,         scipy CubicSpline(bc_type=clamped, 0 slopes); the
+ *   orc_interp,         Gaussian f reproduced; zero outside the knots in
+ *   orc_spline_low_slope  reference mode; the line s u below the first knot  test_oracle_math.py
+ *   orc_wminus1,        hand-computed vectors, Richardson coefficients cancel
+ *   orc_evmean          1/N and 1/N^2 terms, the cosh guard        test_oracle_math.py
+ *   orc_exact, smT*     mpmath; small-T series = Taylor series of Exact   test_oracle_math.py
+ *   orc_igrand          the branch policy's closed-form branches    test_oracle_math.py
+ *   orc_logsimpson      exact for cubics in ln T                    test_oracle_math.py
+ *   orc_field_at,       single-tube window = isolated, periodic in x_cm, cell
+ *   orc_igrand_B        average 2 pi F/(e l^2)                      test_oracle_math.py
+ *   orc_magfield        -1/2 int B^2 = the reference's closed-form ClassAction
+ *                       (step, smooth, quadratic, Gaussian)         test_abi.py
  */
 #include "wfl_oracle.h"
 #include <math.h>
@@ -241,6 +251,9 @@ static double orc_f_mode(const orc_geom *g, int mode, double u) {
   }
   return 0.0;
 }
+
+/* f(rho^2) in a given segment mode (physical mode extends the spline table). */
+double orc_f_in_mode(const orc_geom *g, int mode, double u) { return orc_f_mode(g, mode, u); }
 
 static double orc_fp_mode(const orc_geom *g, int mode, double u) {
   double l2 = g->l2;

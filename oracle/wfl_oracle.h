@@ -57,6 +57,7 @@ double orc_expint(double x);
 double orc_interp(double rho2, const float *rho2sp, const float *coefs, int n);
 /* slope of the physical-mode spline line below the first knot */
 double orc_spline_low_slope(const float *rho2sp, const float *coefs);
+double orc_f_in_mode(const orc_geom *g, int mode, double u);
 
 /* Per-loop sub-loop sums: scal[3] = F*(x, y, z) and ferm[3] as WilsonLoop
  * writes them (intEV.cu:553-560); lattice images summed. */

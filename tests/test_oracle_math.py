@@ -80,6 +80,32 @@ def test_interp_outside_knots_is_zero(orc):
     assert orc.lib.orc_interp(0.7, knots.ctypes.data, coefs.ctypes.data, 4) == pytest.approx(0.2 + 0.2)
 
 
+def test_spline_physical_extension(orc):
+    """Physical mode extends the spline table: f = s u below the first knot
+    with s = fl32(c0/k0) (c1 when the first knot is 0), meeting the first
+    interval's value at k0 to FP32 rounding; f = f(last knot) from the last
+    knot on; the table's own cubic in between (reference mode: 0 outside)."""
+    rho = (np.arange(12) + 1) * 0.25
+    knots = (rho * rho).astype(np.float32)
+    coefs, _ = orc.spline(-np.expm1(-knots.astype(np.float64)), knots)
+    g = orc.Geom(orc.SPLINE, knots=knots, coefs=coefs)
+    k0, kl = float(knots[0]), float(knots[-1])
+    s = float(np.float32(float(coefs[0, 0]) / k0))
+    assert orc.lib.orc_spline_low_slope(knots.ctypes.data, coefs.ctypes.data) == s
+    for u in (1e-6, 0.3 * k0, 0.999 * k0):
+        assert orc.lib.orc_f_in_mode(g.ref, orc.MODE_PHYSICAL, u) == pytest.approx(s * u, rel=1e-15)
+        assert orc.lib.orc_f_in_mode(g.ref, orc.MODE_REFERENCE, u) == 0.0
+    assert s * k0 == pytest.approx(float(coefs[0, 0]), rel=2e-7)            # continuous at k0
+    for u in (kl, 1.5 * kl, 100.0):
+        assert orc.lib.orc_f_in_mode(g.ref, orc.MODE_PHYSICAL, u) == float(coefs[-1, 0])
+    u = 0.5 * (float(knots[4]) + float(knots[5]))
+    assert orc.lib.orc_f_in_mode(g.ref, orc.MODE_PHYSICAL, u) == orc.lib.orc_interp(
+        u, knots.ctypes.data, coefs.ctypes.data, len(knots))
+    zero = np.concatenate([[0.0], knots[1:]]).astype(np.float32)
+    cz = coefs.copy()
+    assert orc.lib.orc_spline_low_slope(zero.ctypes.data, cz.ctypes.data) == float(cz[0, 1])
+
+
 # ---------------------------------------------------------------- spline
 def test_spline_matches_scipy_clamped(orc, wfl):
     n = 64

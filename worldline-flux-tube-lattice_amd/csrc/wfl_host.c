@@ -350,12 +350,24 @@ double *Tfunc(double T, void *p, int *order) {
   wfl_geometry g = g_prof;
   g.F = params->F;
   g.l2 = params->l2;
+  /* The reference calls EV once per group (Integrand.c:249-301); every call
+   * evaluates (or here reuses) the same launch and rewrites Wsscal_h/Wsferm_h
+   * with the same values.  After the first successful call of this T the
+   * host buffers already hold them, so the other groups only average: the
+   * same numbers and the same final buffer contents. */
+  int filled = 0;
   for (int i = 0; i < params->ng; ++i) {
     int branch;
     double v = igrand_geom(&g, params->fermion, T, rho, rho2, 0.0, &branch);
     if (branch == 3) {
       for (int j = 0; j < groupsize; ++j) WLlist[j] = order[j + i * groupsize];
-      double ev = EV(T, p, WLlist);
+      double ev = 0.0;
+      if (filled) {
+        EVMean(&ev, params->Wsscal_h, params->Wsferm_h, groupsize, WLlist, T, params->fermion);
+      } else {
+        ev = EV(T, p, WLlist);
+        filled = !isnan(ev);
+      }
       v = igrand_geom(&g, params->fermion, T, rho, rho2, ev - 1.0, &branch);
     }
     igrand[i] = v;

@@ -171,6 +171,13 @@ int wfl_plan_accurate_tuples(const wfl_plan *p);
  * r = sqrt(rho^2) (the layout of periodic.c:129-151), checked to 1e-4 of
  * the spacing; 0 when the kernels use the exact interval search. */
 int wfl_plan_uniform_splines(const wfl_plan *p);
+/* 1 when the lane-per-tuple kernels of a mixed plan use the axis form:
+ * physical mode, Simpson n = 2, no lattice geometry and every tuple's x_cm
+ * on the x axis (cy = 0), so that only z_x = c + s w_x is formed per
+ * point and rho^2 = z_x^2 + s^2 w_y^2, z x v = z_x v_y - s (w_y v_x) take
+ * their y parts from loop-only point records; the same nodes and sums as
+ * intEV.cu:509-571 (the reference forms z and its products per point). */
+int wfl_plan_axis_form(const wfl_plan *p);
 /* Simpson sub-steps chosen by measured accuracy (the batch's own nsub is
  * ignored): the smallest n in 2, 4, 8, ... <= nmax whose group sums on a
  * sample of the batch (every k-th tuple, at most max_tuples; the first

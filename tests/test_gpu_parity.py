@@ -85,6 +85,56 @@ def test_gaussian_mixed(gpu, orc, loops, mapping, mode, spinor):
     assert e1 < 2e-5 and e2 < 1e-5
 
 
+@pytest.mark.parametrize("spinor", [0, 1])
+def test_axis_form(gpu, orc, loops, spinor):
+    """Mixed physical n = 2 batches whose tuples all have x_cm on the x axis
+    take the axis form (rho^2 and the cross products from loop-only
+    records); the same tuples with cy = 1e-30 (the same problem to FP32)
+    take the general form.  Both meet the mixed bars against the oracle, and
+    the axis form's error stays within a small multiple of the general
+    form's: the Gaussian tube (with small loops in the core, the accurate
+    node), a uniform-grid spline table, and the run-time profile kernels
+    (smooth), under each mapping.  A lattice geometry, a tuple off the axis,
+    n = 4 and reference mode keep the general form."""
+    knots, coefs = gaussian_spline(gpu)
+    cases = [
+        ([gpu.Geometry(kind=gpu.GAUSSIAN, F=1.0, l2=1.0)], [orc.Geom(orc.GAUSSIAN, F=1.0, l2=1.0)], ()),
+        ([gpu.Geometry(kind=gpu.SPLINE, F=1.0, l2=1.0, a=1.0, spline_table=0)],
+         [orc.Geom(orc.SPLINE, F=1.0, l2=1.0, tubedist=1.0, knots=knots, coefs=coefs)], [(knots, coefs)]),
+        ([gpu.Geometry(kind=gpu.SMOOTH, F=0.7, l2=0.5)], [orc.Geom(orc.SMOOTH, F=0.7, l2=0.5)], ()),
+    ]
+    for geoms, og, spl in cases:
+        for mapping in (1, 2, 3):
+            res = {}
+            for cy in (0.0, 1e-30):
+                seen = []
+                gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, 1, 2, spinor, gpu.MIXED, mapping,
+                                            tuples(GAUSS_RHO, GAUSS_T, cy=cy), splines=spl,
+                                            on_plan=lambda p: seen.append(p.axis_form))
+                res[cy] = (relerr(pl, rpl), relerr(gs, rgs), seen[0])
+            (e1a, e2a, ax), (e1g, e2g, gen) = res[0.0], res[1e-30]
+            print(f"  profile {geoms[0].kind} mapping {mapping}: axis perloop {e1a:.2e} gsum {e2a:.2e}; "
+                  f"general perloop {e1g:.2e} gsum {e2g:.2e}")
+            assert ax == 1 and gen == 0
+            assert e1a < 2e-5 and e2a < 1e-5 and e1g < 2e-5 and e2g < 1e-5
+            assert e1a < 4.0 * e1g + 1e-7 and e2a < 4.0 * e2g + 1e-7
+    # batches that keep the general form
+    g0 = [gpu.Geometry(kind=gpu.GAUSSIAN, F=1.0, l2=1.0)]
+    o0 = [orc.Geom(orc.GAUSSIAN, F=1.0, l2=1.0)]
+    tg, tcx, tcy, tT = tuples(GAUSS_RHO, GAUSS_T)
+    tcy_off = tcy.copy()
+    tcy_off[-1] = 0.3
+    lat = [gpu.Geometry(kind=gpu.GAUSSIAN, F=1.0, l2=0.25, lattice=1.5, images=1)]
+    olat = [orc.Geom(orc.GAUSSIAN, F=1.0, l2=0.25, lattice=1.5, images=1)]
+    for geoms, og, mode, nsub, tup in ((g0, o0, 1, 2, (tg, tcx, tcy_off, tT)), (g0, o0, 1, 4, (tg, tcx, tcy, tT)),
+                                       (g0, o0, 0, 2, (tg, tcx, tcy, tT)), (lat, olat, 1, 2, (tg, tcx, tcy, tT))):
+        seen = []
+        gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, mode, nsub, spinor, gpu.MIXED, 1, tup,
+                                    on_plan=lambda p: seen.append(p.axis_form))
+        assert seen == [0]
+        assert relerr(pl, rpl) < 2e-5 and relerr(gs, rgs) < 1e-5
+
+
 @pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 2e-5)])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_nsub50_and_multi_geometry(gpu, orc, loops, prec, tol, mode):
@@ -305,12 +355,15 @@ def test_other_profiles_mixed_small_loops_at_centre(gpu, orc, loops, kind, spino
     reference's forms of the periodic and fixflux profiles are 1 minus nearly
     1 there (relative error ~1e-7 / f in FP32, ~1e-3 in the phase at
     T = 1e-3); the FP32 kernels use equivalent forms without the
-    cancellation and keep the per-loop phases within 2e-6 (for the fixflux
+    cancellation and keep the per-loop phases within 4e-6 (for the fixflux
     form, the reference's phi(0) = 1 - q2*/q2 of the rounded q2 included).
     The periodic rings sit at a = 10 lambda: nearer rings leave f(0) > 0 (the
     sum of their tails), a point flux whose pure-gauge part small loops
     beside the axis cancel in FP32 like a distant tube's.  Loops further out
-    (|x_cm|/sqrt(T) ~ 10) carry the symmetric gauge's ~1e-7 |x_cm|/sqrt(T)."""
+    (|x_cm|/sqrt(T) ~ 10) carry the symmetric gauge's ~1e-7 |x_cm|/sqrt(T).
+    These tuples lie on the x axis (the axis form); nodes next to the axis
+    set the error through the FP32 roundings of rho^2 and the cross products
+    (2e-7 to 2e-6 with either form's roundings)."""
     k = getattr(gpu, kind)
     a = 8.0 if kind == "PERIODIC" else 2.0
     geoms = [gpu.Geometry(kind=k, F=0.5, l2=0.64, a=a, lam_min=0.2)]
@@ -323,7 +376,7 @@ def test_other_profiles_mixed_small_loops_at_centre(gpu, orc, loops, kind, spino
         e_spin = relerr(pl[t, :, 3:], rpl[t, :, 3:]) if spinor else 0.0
         print(f"  {kind} tuple {t}: phase {e_phase:.2e} spin {e_spin:.2e}")
         errs.append(max(e_phase, e_spin))
-    assert max(errs) < 2e-6, errs
+    assert max(errs) < 4e-6, errs
 
 
 @pytest.mark.parametrize("spinor", [0, 1])

@@ -90,6 +90,31 @@ __global__ void k_records(const float* __restrict__ x, const float* __restrict__
   r[5] = (R)((double)yl[s] - (double)yl[p]);
 }
 
+// Axis-form records (see wfl_kernels.h): w_x, w_y^2, 4 dw_x, m_y^2 with
+// m_y = (w_y + w_y,succ)/2, 4 dw_y, e_y, w_y 4 dw_x, w_y e_x; differences
+// of FP32 inputs (exact in double) and products rounded once.
+__global__ void k_records_axis(const float* __restrict__ x, const float* __restrict__ y, int Nl, int N,
+                               float* __restrict__ rec) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)Nl * N) return;
+  int l = (int)(idx / N), i = (int)(idx % N);
+  const float* xl = x + (size_t)l * N;
+  const float* yl = y + (size_t)l * N;
+  int s = succ_idx(i, N), p = pred_idx(i, N);
+  const double wx = xl[i], wy = yl[i];
+  const double dx4 = 4.0 * ((double)xl[s] - wx), my = 0.5 * (wy + (double)yl[s]);
+  const double ex = (double)xl[s] - (double)xl[p];
+  float* r = rec + (size_t)idx * RECA;
+  r[0] = (float)wx;
+  r[1] = (float)(wy * wy);
+  r[2] = (float)dx4;
+  r[3] = (float)(my * my);
+  r[4] = (float)(4.0 * ((double)yl[s] - wy));
+  r[5] = (float)((double)yl[s] - (double)yl[p]);
+  r[6] = (float)(wy * dx4);
+  r[7] = (float)(wy * ex);
+}
+
 // Largest |w|^2 over all points: block maximum, then one atomic per block
 // on the float's bits (non-negative floats order like their bit patterns).
 __global__ __launch_bounds__(256) void k_max_radius2(const float* __restrict__ x, const float* __restrict__ y,
@@ -188,6 +213,7 @@ struct wfl_loops_s {
   float* y = nullptr;
   float* rec32 = nullptr;  // [Nl][N][REC]
   double* rec64 = nullptr;
+  float* reca = nullptr;   // axis form [Nl][N][RECA]
   float rmax = -1.0f;      // largest |w| over the ensemble (< 0: not yet measured)
 };
 
@@ -221,6 +247,7 @@ extern "C" void wfl_loops_destroy(wfl_loops* L) {
   if (L->y) (void)hipFree(L->y);
   if (L->rec32) (void)hipFree(L->rec32);
   if (L->rec64) (void)hipFree(L->rec64);
+  if (L->reca) (void)hipFree(L->reca);
   delete L;
 }
 
@@ -316,6 +343,17 @@ static int loops_records(wfl_loops* L, int precision) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipDeviceSynchronize());
   }
+  return WFL_OK;
+}
+
+static int loops_records_axis(wfl_loops* L) {
+  if (L->reca) return WFL_OK;
+  long long n = (long long)L->Nl * L->N;
+  HIP_TRY(hipMalloc(&L->reca, sizeof(float) * RECA * (size_t)n));
+  hipLaunchKernelGGL(k_records_axis, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, L->x, L->y, L->Nl, L->N,
+                     L->reca);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipDeviceSynchronize());
   return WFL_OK;
 }
 
@@ -628,6 +666,7 @@ struct wfl_plan_s {
   float4* d_sblob = nullptr;     // spline blob
   int sblob_f4 = 0, lds_splines = 0;
   int uni_splines = 0, sblob_uni_f4 = 0;   // uniform-grid spline rows (FP32 physical)
+  int axis = 0;                            // axis-form records (see wfl_plan_axis_form)
   double* d_gsum = nullptr;
   double* d_perloop = nullptr;
   double* d_wm1 = nullptr;
@@ -838,6 +877,15 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
   p->mapping = mapping;
 
   TRY(loops_records(L, b->precision));
+  // the axis form: mixed physical n = 2 batches whose tuples all have
+  // x_cm on the x axis and no lattice geometry
+  {
+    bool ax = b->precision == WFL_MIXED && b->mode == WFL_MODE_PHYSICAL && b->nsub == 2 && b->ntup > 0;
+    for (int g = 0; g < b->ngeom && ax; ++g) ax = !(b->geoms[g].lattice > 0.0);
+    for (int t = 0; t < b->ntup && ax; ++t) ax = p->tups[t].cy == 0.0f;
+    p->axis = ax ? 1 : 0;
+    if (p->axis) TRY(loops_records_axis(L));
+  }
   TRY(upload(&p->d_tups, p->tups.data(), p->tups.size()));
   TRY(upload(&p->d_geoms, geoms.data(), geoms.size()));
   TRY(upload(&p->d_lanes, lanes.data(), lanes.size()));
@@ -890,6 +938,7 @@ static void fill_launch(const wfl_plan* p, WflLaunch& a, hipStream_t s) {
   a.nslots = p->mapping == 1 ? p->ngroups * p->cpg : p->ngroups;
   a.sblob = p->d_sblob; a.sblob_f4 = p->sblob_f4; a.lds_splines = p->lds_splines;
   a.uni_splines = p->uni_splines; a.sblob_uni_f4 = p->sblob_uni_f4;
+  a.axis = p->axis; a.rec_axis = p->L->reca;
   a.prof_gauss = p->prof_gauss; a.spinor = p->b.spinor ? 1 : 0; a.mode = p->b.mode; a.nsub = p->b.nsub;
   a.split = p->mapping == 3;
   a.gsum = (p->mapping == 1 && p->cpg > 1) ? p->d_gpart : p->d_gsum;
@@ -1000,7 +1049,7 @@ extern "C" int wfl_plan_autotune(wfl_plan* p, void* stream, int* chosen) {
   memset(&key, 0, sizeof key);
   key.ntup = p->b.ntup; key.Nl = p->L->Nl; key.N = p->L->N; key.precision = p->b.precision;
   key.spinor = p->b.spinor; key.mode = p->b.mode; key.nsub = p->b.nsub; key.prof_gauss = p->prof_gauss;
-  key.lds_splines = p->lds_splines + 2 * p->uni_splines;
+  key.lds_splines = p->lds_splines + 2 * p->uni_splines + 4 * p->axis;
   key.ntasks1 = p->ntasks1;
   {
     bool hit = false;
@@ -1126,6 +1175,8 @@ extern "C" int wfl_plan_accurate_tuples(const wfl_plan* p) {
   return n;
 }
 extern "C" int wfl_plan_uniform_splines(const wfl_plan* p) { return p ? p->uni_splines : 0; }
+
+extern "C" int wfl_plan_axis_form(const wfl_plan* p) { return p ? p->axis : 0; }
 extern "C" const double* wfl_plan_gsum_device(const wfl_plan* p) { return p ? p->d_gsum : nullptr; }
 extern "C" int wfl_plan_mapping(const wfl_plan* p) { return p ? p->mapping : 0; }
 

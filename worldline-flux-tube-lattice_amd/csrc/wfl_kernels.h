@@ -48,6 +48,16 @@ __device__ __forceinline__ int pred_idx(int i, int N) {
 // accumulator slot of point i: 0 = x (odd), 1 = y (2 mod 4), 2 = z (0 mod 4)
 __device__ __forceinline__ int cls_of(int i) { return (i & 1) ? 0 : ((i & 3) == 0 ? 2 : 1); }
 
+// Axis-form records (M_AXIS), 8 floats per point, loop-only:
+//   (w_x, w_y^2, 4 dw_x, m_y^2, 4 dw_y, e_y, w_y 4 dw_x, w_y e_x),  m = w + dw/2.
+// With x_cm = (c, 0) only the x coordinate mixes c and s w, so a tuple
+// (s = sqrt(T)) forms z_x = c + s w_x and the midpoint's x as in point_eval
+// (the same roundings) and takes the rest from the record:
+//   rho^2 = z_x^2 + s^2 w_y^2                       (midpoint: m_x, m_y)
+//   z x v = z_x v_y - s (w_y v_x)                   (v = 4 dw, e)
+// 11 VALU per point besides the nodes (point_eval: 14).
+constexpr int RECA = 8;
+
 // Per-lane constants of one tuple in the kernel's type and units.
 template <class R> struct LaneC {
   R s;          // sqrt(T)
@@ -146,6 +156,32 @@ __device__ __forceinline__ void point_eval(R wx, R wy, R dx4, R dy4, R ex, R ey,
   }
 }
 
+// Lane constants of the axis form: c and s^2 (FP32, s^2 rounded once).
+struct LaneA {
+  float c, a2;
+};
+
+// One point of the axis form (physical mode, n = 2): the same nodes, weights
+// and sums as point_eval, with z_x and the midpoint's x rounded as there (so
+// rho^2 and the cross products stay consistent where a loop passes the
+// axis), the y parts from the axis record r.  Pz collects z_x (e_y g_A +
+// 4 dw_y g_mid), Ps the w_y parts; the caller adds Pz - s Ps in FP64.
+template <int PROF, int SPINOR, int ACC>
+__device__ __forceinline__ void point_eval_axis(const float* __restrict__ r, const LaneA& la,
+                                                const LaneC<float>& lc, int kind, const GP<float>& gp,
+                                                const SplineRef& sp, float& Pz, float& Ps, float& S) {
+  const float zx = fmaf(lc.s, r[0], la.c);
+  const float zm = fmaf(lc.m8, r[2], zx);
+  const float A = fmaf(zx, zx, fmaf(la.a2, r[1], 1e-30f));
+  const float Am = fmaf(zm, zm, fmaf(la.a2, r[3], 1e-30f));
+  float fA, fk;
+  const float gA = node_g<float, PROF, ACC>(A, lc, kind, gp, M_PHYSICAL, sp, SPINOR, fA);
+  const float gm = node_g<float, PROF, ACC>(Am, lc, kind, gp, M_PHYSICAL, sp, SPINOR, fk);
+  Pz = fmaf(zx, fmaf(r[5], gA, r[4] * gm), Pz);
+  Ps = fmaf(r[7], gA, fmaf(r[6], gm, Ps));
+  if (SPINOR) S += fA + (fk + fk);
+}
+
 // (a library test: v - v == 0 is not one once the compiler contracts v = a b
 // into fma(a, b, -v), the rounding error of the product)
 template <class R> __device__ __forceinline__ bool finite_r(R v) { return __builtin_isfinite(v); }
@@ -233,6 +269,28 @@ template <class R, int PROF, int SPINOR, int MODE, int NSUB, int ACC>
 __device__ __forceinline__ void image_sums(const R* __restrict__ rl, int N, R ccx, R ccy, const LaneC<R>& lc,
                                            int kind, const GP<R>& gp, const SplineRef& sp, int nsub_rt,
                                            double P[3], double S[3]) {
+  if constexpr (MODE == M_AXIS) {
+    static_assert(sizeof(R) == 4 && NSUB == 2, "the axis form is FP32 with n = 2");
+    const LaneA la = {ccx, lc.s * lc.s};
+    for (int i0 = 0; i0 < N; i0 += 32) {
+      float pz[3] = {0.0f, 0.0f, 0.0f}, ps[3] = {0.0f, 0.0f, 0.0f}, q[3] = {0.0f, 0.0f, 0.0f};
+      const int i1 = i0 + 32 < N ? i0 + 32 : N;
+      for (int i = i0; i < i1; i += 4) {
+        const float* r = rl + (size_t)i * RECA;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int c = (u & 1) ? 0 : (u == 0 ? 2 : 1);
+          point_eval_axis<PROF, SPINOR, ACC>(r + RECA * u, la, lc, kind, gp, sp, pz[c], ps[c], q[c]);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        P[c] += fma(-(double)lc.s, (double)ps[c], (double)pz[c]);
+        if (SPINOR) S[c] += (double)q[c];
+      }
+    }
+    return;
+  }
   for (int i0 = 0; i0 < N; i0 += 32) {
     R p[3] = {(R)0, (R)0, (R)0}, q[3] = {(R)0, (R)0, (R)0};
     const int i1 = i0 + 32 < N ? i0 + 32 : N;
@@ -324,15 +382,16 @@ __device__ __forceinline__ void tuple_task(
   const SplineRef sp = spline_ref(sblob, gm);
   const int n = NSUB ? NSUB : nsub_rt;
   // image window (uniform over the lane group: one geometry per group)
-  const int M = __builtin_amdgcn_readfirstlane(gm.lattice > 0.0 ? gm.images : 0);
+  const int M = MODE == M_AXIS ? 0 : __builtin_amdgcn_readfirstlane(gm.lattice > 0.0 ? gm.images : 0);
   const R lat = sizeof(R) == 4 ? (R)gm.latticef : (R)gm.lattice;
   const R cx = sizeof(R) == 4 ? (R)tp.cx : (R)tp.cxd, cy = sizeof(R) == 4 ? (R)tp.cy : (R)tp.cyd;
 
-  const bool lattice = gm.lattice > 0.0;
+  const bool lattice = MODE != M_AXIS && gm.lattice > 0.0;
+  constexpr int RS = MODE == M_AXIS ? RECA : REC;   // record stride
 
   double G[3] = {0.0, 0.0, 0.0};
   for (int l = loop0; l < loop0 + nloop; ++l) {
-    const R* __restrict__ rl = (const R*)__builtin_assume_aligned(rec + (size_t)l * N * REC, 2 * sizeof(R));
+    const R* __restrict__ rl = (const R*)__builtin_assume_aligned(rec + (size_t)l * N * RS, 2 * sizeof(R));
     double P[3] = {0.0, 0.0, 0.0}, S[3] = {0.0, 0.0, 0.0};
     loop_sums<R, PROF, SPINOR, MODE, NSUB>(rl, N, M, lattice, tp, gm, cx, cy, lat, lc, kind, gp, sp, nsub_rt, P, S);
     double sc[3], fe[3];
@@ -499,6 +558,8 @@ __global__ __launch_bounds__(256) void k_loop(
 // Launch interface of one precision's kernel instantiations.
 struct WflLaunch {
   const void* rec; int N, Nl;
+  const float* rec_axis;                      // axis-form records (FP32, lane-per-tuple kernels)
+  int axis;                                   // 1: every tuple on the x axis, no lattice, n = 2
   const wfl::DevTuple* tups; const wfl::DevGeom* geoms; int ntup;
   const int* lanes; const wfl::Task* tasks; int ntasks; int nslots;
   const float4* sblob; int sblob_f4;          // spline blob and its size in float4

@@ -14,6 +14,10 @@ namespace wfl {
 enum { P_STEP = 0, P_SMOOTH = 1, P_QUADRATIC = 2, P_GAUSSIAN = 3,
        P_PERIODIC = 4, P_SPLINE = 5, P_FIXFLUX = 6, P_FIXFLUX_FF = 7 };
 enum { M_REFERENCE = 0, M_PHYSICAL = 1 };
+// Kernel-only segment mode: physical mode for launches whose tuples all
+// have x_cm on the x axis and no lattice images (FP32, Simpson n = 2), with
+// the loop-only point records of the axis form (wfl_kernels.h).
+enum { M_AXIS = 2 };
 // kernel tag for "profile chosen at run time from DevGeom::kind"
 enum { P_ANY = 99 };
 // Kernel-only profile: a spline whose knots lie on a uniform grid in

@@ -367,6 +367,7 @@ def run_config(wfl, args, mode, cfg):
     gs_m, _ = mixed.read()
     out["nonfinite_loops_mixed"] = mixed.nonfinite()
     out["accurate_node_tuples"] = mixed.accurate_tuples
+    out["axis_form"] = mixed.axis_form
     if cfg.get("splines"):
         out["uniform_grid_splines"] = mixed.uniform_splines
     mixed.close()
@@ -627,6 +628,7 @@ def bench_c2(wfl, args, mode):
     mixed = r.plan(wfl.MIXED)
     mapping_used = mixed.mapping
     accurate = mixed.accurate_tuples
+    axis = mixed.axis_form
     ms_step, k_mixed = r.timed(mixed, args.steps, args.warmup, "mixed")
     gs_mixed, _ = mixed.read()
     nonfinite = mixed.nonfinite()
@@ -653,7 +655,7 @@ def bench_c2(wfl, args, mode):
     achieved = fl * pe_step / k_mixed
     result["roofline"] = {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32_VECTOR / 1e12,
                           "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_VECTOR, "traffic": None,
-                          "kernel": "k_tuple<float,GAUSSIAN,spinor,physical,n=2>",
+                          "kernel": f"k_tuple<float,GAUSSIAN,spinor,{'axis' if axis else 'physical'},n=2>",
                           "kernel_ms": k_mixed * 1e3, "flops_per_point_eval": fl}
     tr = os.path.join(ROOT, "profiles", f"traffic_{cfg['name']}.json")
     if os.path.exists(tr):
@@ -661,6 +663,7 @@ def bench_c2(wfl, args, mode):
             result["roofline"]["traffic"] = json.load(fh).get("hbm_bytes_per_launch")
     result["nonfinite_loops"] = nonfinite
     result["accurate_node_tuples"] = accurate
+    result["axis_form"] = axis
     if not args.no_fp64:
         fp64 = r.plan(wfl.FP64)
         ms64, k64 = r.timed(fp64, args.fp64_steps, 0, "fp64")

@@ -763,8 +763,8 @@ static int plan_set_tasks(wfl_plan* p, int cpg) {
 // it close together and stay within a few loops of each other, and its
 // point records are reused from L2 instead of being fetched once per lane
 // group (C2, 128-loop groups: 700 ms with one task per group, 649 ms with
-// 8-loop tasks, 646 ms with 4).  The chunk sums cost 24 B per tuple, group
-// and chunk; they are kept under 2 GB.
+// 16-loop tasks, 647 to 649 ms with 8, 646 ms with 4).  The chunk sums cost
+// 24 B per tuple, group and chunk; they are kept under 2 GB.
 static const int kTaskLoops = 8;
 
 static int plan_default_cpg(const wfl_plan* p, int mapping) {

@@ -367,6 +367,7 @@ def run_config(wfl, args, mode, cfg):
     gs_m, _ = mixed.read()
     out["nonfinite_loops_mixed"] = mixed.nonfinite()
     out["accurate_node_tuples"] = mixed.accurate_tuples
+    out["local_node_tuples"] = mixed.local_tuples
     out["axis_form"] = mixed.axis_form
     if cfg.get("splines"):
         out["uniform_grid_splines"] = mixed.uniform_splines
@@ -628,6 +629,7 @@ def bench_c2(wfl, args, mode):
     mixed = r.plan(wfl.MIXED)
     mapping_used = mixed.mapping
     accurate = mixed.accurate_tuples
+    local = mixed.local_tuples
     axis = mixed.axis_form
     ms_step, k_mixed = r.timed(mixed, args.steps, args.warmup, "mixed")
     gs_mixed, _ = mixed.read()
@@ -663,6 +665,7 @@ def bench_c2(wfl, args, mode):
             result["roofline"]["traffic"] = json.load(fh).get("hbm_bytes_per_launch")
     result["nonfinite_loops"] = nonfinite
     result["accurate_node_tuples"] = accurate
+    result["local_node_tuples"] = local
     result["axis_form"] = axis
     if not args.no_fp64:
         fp64 = r.plan(wfl.FP64)

@@ -178,6 +178,21 @@ int wfl_plan_uniform_splines(const wfl_plan *p);
  * their y parts from loop-only point records; the same nodes and sums as
  * intEV.cu:509-571 (the reference forms z and its products per point). */
 int wfl_plan_axis_form(const wfl_plan *p);
+/* Tuples of a Gaussian axis-form plan that take the local Taylor node:
+ * loops small against their distance from the tube centre (every node
+ * within 0.5 of uc = c'^2 in the scaled units u' = log2(e) rho^2 / l2, for
+ * loops up to the 90th percentile of the ensemble's loop radii; the kernel
+ * checks each loop) evaluate (1 - 2^-u')/u' as g(uc) + sum_k a_k (u' - uc)^k,
+ * k = 1..7, with the g(uc) part of every sub-loop sum taken exactly from
+ * per-loop FP64 area sums.  A mixed-precision accuracy measure with no
+ * counterpart in the reference (intEV.cu evaluates the node directly).
+ * The lane-per-tuple mappings (1, 3) carry it; mapping 0 and the autotuner
+ * keep to them when a plan has such tuples, and an explicit mapping 2
+ * evaluates those tuples with the direct node. */
+int wfl_plan_local_tuples(const wfl_plan *p);
+/* The local node's coefficients a[k] = g^(k)(uc)/k!, k = 0..7, of
+ * g(u) = (1 - 2^-u)/u, in FP64. */
+int wfl_local_taylor(double uc, double *a);
 /* Simpson sub-steps chosen by measured accuracy (the batch's own nsub is
  * ignored): the smallest n in 2, 4, 8, ... <= nmax whose group sums on a
  * sample of the batch (every k-th tuple, at most max_tuples; the first

@@ -274,3 +274,34 @@ def test_evmean_matches_oracle(wfl, orc):
     out = C.c_double(0.0)
     wfl.lib.EVMean(C.addressof(out), ws.ctypes.data, wf.ctypes.data, 128, WL.ctypes.data, 0.7, 1)
     assert out.value == pytest.approx(orc.evmean(ws, wf, WL, 1), rel=1e-13)
+
+
+def test_local_taylor_coefficients(wfl):
+    """The local Taylor node's coefficients a_k = g^(k)(uc)/k! of
+    g(u) = (1 - 2^-u)/u against high-precision derivatives (mpmath), and
+    the series' truncation at k = 7 within 4e-9 of g(u) - g(uc) for
+    |u - uc| <= 0.5 (the kernel's range) from uc = 0 to 9."""
+    import mpmath as mp
+    mp.mp.dps = 40
+    L = mp.log(2)
+
+    def g(u):
+        u = mp.mpf(u)
+        return (1 - mp.power(2, -u)) / u if u != 0 else L
+
+    for uc in (0.0, 1e-3, 0.3, 1.0, 2.25, 4.0, 9.0):
+        a = np.zeros(8)
+        wfl.check(wfl.lib.wfl_local_taylor(uc, a.ctypes.data))
+        u0 = mp.mpf(uc) if uc > 0 else mp.mpf("1e-30")
+        for k in range(8):
+            ref = mp.diff(g, u0, k) / mp.factorial(k)
+            assert abs(float((a[k] - ref) / ref)) < 1e-11, (uc, k)
+        worst = 0.0
+        for d in np.linspace(-0.5, 0.5, 41):
+            if d == 0.0 or uc + d <= 0.0:
+                continue
+            exact = g(uc + d) - g(uc)
+            series = sum(mp.mpf(a[k]) * mp.mpf(d) ** k for k in range(1, 8))
+            worst = max(worst, abs(float((series - exact) / exact)))
+        assert worst < 4e-9, (uc, worst)
+    assert wfl.lib.wfl_local_taylor(-1.0, np.zeros(8).ctypes.data) == -1

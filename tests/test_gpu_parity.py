@@ -416,3 +416,33 @@ def test_gaussian_mixed_small_loops_lattice(gpu, orc, loops):
         assert relerr(pl[t, :, 3:], rpl[t, :, 3:]) < 2e-6, t
     print(f"  max phase err {max(relerr(pl[t, :, :3], rpl[t, :, :3]) for t in range(pl.shape[0])):.2e}")
     print(f"  max spin err {max(relerr(pl[t, :, 3:], rpl[t, :, 3:]) for t in range(pl.shape[0])):.2e}")
+
+
+@pytest.mark.parametrize("spinor", [0, 1])
+def test_gaussian_mixed_local_node(gpu, orc, loops, spinor):
+    """Small loops away from the axis of a wide tube (lambda = 2, x_cm up
+    to 1.5 lambda, T = 1e-3 .. 3e-2) take the local Taylor node of the axis
+    form: g(u) expanded about the loop centre's u, the pure-gauge part of the
+    sub-loop sums from per-loop FP64 area sums, the spin node as 2^-uc minus
+    a small deficit.  Per-loop phases within 2e-6 of the oracle (relative to
+    the tuple's largest; the direct node reaches 1.4e-5 on these tuples, and
+    a lane whose own loop is too large for the series takes it for that
+    loop), spin sums within 1e-7, and the local node serves every tuple of
+    the case."""
+    geoms = [gpu.Geometry(kind=gpu.GAUSSIAN, F=1.0, l2=4.0)]
+    og = [orc.Geom(orc.GAUSSIAN, F=1.0, l2=4.0)]
+    seen = []
+    tg, tcx, tcy, tT = tuples([0.0, 0.4, 1.14, 2.0], [1e-3, 3e-3, 1e-2, 3e-2])
+    t3 = tuples([3.0], [1e-3, 3e-3, 1e-2])
+    tup = tuple(np.concatenate([a, b]) for a, b in zip((tg, tcx, tcy, tT), t3))
+    gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, 1, 2, spinor, gpu.MIXED, 1, tup,
+                                on_plan=lambda p: seen.append(p.local_tuples))
+    assert seen == [19]
+    e_ph = max(relerr(pl[t, :, :3], rpl[t, :, :3]) for t in range(pl.shape[0]))
+    print(f"  max phase err {e_ph:.2e}")
+    assert e_ph < 2e-6
+    if spinor:
+        e_sp = max(relerr(pl[t, :, 3:], rpl[t, :, 3:]) for t in range(pl.shape[0]))
+        print(f"  max spin err {e_sp:.2e}")
+        assert e_sp < 1e-7
+    assert relerr(gs, rgs) < 1e-6

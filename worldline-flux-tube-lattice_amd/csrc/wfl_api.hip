@@ -132,6 +132,47 @@ __global__ __launch_bounds__(256) void k_max_radius2(const float* __restrict__ x
   if (threadIdx.x == 0) atomicMax(out, __float_as_uint(red[0]));
 }
 
+// Per-loop constants of the local Taylor node: class sums
+// Q_c = sum_{i in c} w_i x (e_i + 4 dw_i) and the largest radius, in FP64
+// from the FP32 coordinates (products of differences of FP32 values).  One
+// block per loop; each thread sums a fixed stride of points and the block
+// adds the partial sums in a fixed order.
+__global__ __launch_bounds__(256) void k_loop_info(const float* __restrict__ x, const float* __restrict__ y, int N,
+                                                   LoopInfo* __restrict__ info) {
+  __shared__ double red[4][256];
+  const int l = blockIdx.x;
+  const float* xl = x + (size_t)l * N;
+  const float* yl = y + (size_t)l * N;
+  double q[3] = {0.0, 0.0, 0.0}, r2 = 0.0;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const int sI = succ_idx(i, N), pI = pred_idx(i, N);
+    const double wx = xl[i], wy = yl[i];
+    const double vx = ((double)xl[sI] - (double)xl[pI]) + 4.0 * ((double)xl[sI] - wx);
+    const double vy = ((double)yl[sI] - (double)yl[pI]) + 4.0 * ((double)yl[sI] - wy);
+    const double a = wx * vy - wy * vx;
+    const int c = cls_of(i);
+    q[0] += c == 0 ? a : 0.0;
+    q[1] += c == 1 ? a : 0.0;
+    q[2] += c == 2 ? a : 0.0;
+    r2 = fmax(r2, wx * wx + wy * wy);
+  }
+  red[0][threadIdx.x] = q[0]; red[1][threadIdx.x] = q[1]; red[2][threadIdx.x] = q[2]; red[3][threadIdx.x] = r2;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) {
+      for (int c = 0; c < 3; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + k];
+      red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][threadIdx.x + k]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    LoopInfo o;
+    o.q[0] = red[0][0]; o.q[1] = red[1][0]; o.q[2] = red[2][0];
+    o.r = sqrt(red[3][0]) * (1.0 + 1e-6);
+    info[l] = o;
+  }
+}
+
 __global__ void k_aos_to_soa(const float4* __restrict__ aos, long long n, float* __restrict__ x, float* __restrict__ y) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -215,6 +256,8 @@ struct wfl_loops_s {
   double* rec64 = nullptr;
   float* reca = nullptr;   // axis form [Nl][N][RECA]
   float rmax = -1.0f;      // largest |w| over the ensemble (< 0: not yet measured)
+  LoopInfo* info = nullptr;          // per-loop constants of the local Taylor node [Nl]
+  std::vector<double> radius;        // host copy of the per-loop largest radii
 };
 
 static int loops_alloc(wfl_loops** out, int Nl, int N) {
@@ -248,6 +291,7 @@ extern "C" void wfl_loops_destroy(wfl_loops* L) {
   if (L->rec32) (void)hipFree(L->rec32);
   if (L->rec64) (void)hipFree(L->rec64);
   if (L->reca) (void)hipFree(L->reca);
+  if (L->info) (void)hipFree(L->info);
   delete L;
 }
 
@@ -355,6 +399,75 @@ static int loops_records_axis(wfl_loops* L) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipDeviceSynchronize());
   return WFL_OK;
+}
+
+static int loops_info(wfl_loops* L) {
+  if (L->info) return WFL_OK;
+  HIP_TRY(hipMalloc(&L->info, sizeof(LoopInfo) * (size_t)L->Nl));
+  hipLaunchKernelGGL(k_loop_info, dim3((unsigned)L->Nl), dim3(256), 0, 0, L->x, L->y, L->N, L->info);
+  HIP_TRY(hipGetLastError());
+  std::vector<LoopInfo> h((size_t)L->Nl);
+  HIP_TRY(hipMemcpy(h.data(), L->info, sizeof(LoopInfo) * h.size(), hipMemcpyDeviceToHost));
+  L->radius.resize(h.size());
+  for (size_t l = 0; l < h.size(); ++l) L->radius[l] = h[l].r;
+  return WFL_OK;
+}
+
+// ------------------------------------------------------ local Taylor node
+// a[k] = g^(k)(uc)/k!, k = 0..7, of g(u) = (1 - 2^-u)/u (the FP32 Gaussian
+// node in scaled units): g^(k)(u) = ln2 (-ln2)^k I_k(ln2 u) with
+// I_k(b) = int_0^1 t^k e^(-b t) dt.  I_7 from its series
+// sum_j (-b)^j / (j! (8 + j)), the others by the backward recurrence
+// I_(k-1) = (b I_k + e^-b)/k (integration by parts; stable downwards).
+static const int kLocalK = 7;
+static void local_taylor(double uc, double a[kLocalK + 1]) {
+  const double L = 0.69314718055994530942, b = L * uc;
+  double I[kLocalK + 1];
+  double term = 1.0, sum = 0.0;
+  for (int j = 0; j < 1000; ++j) {
+    sum += term / (double)(kLocalK + j + 1);
+    term *= -b / (double)(j + 1);
+    if ((double)j > b && fabs(term) < 1e-20 * fabs(sum)) break;
+  }
+  I[kLocalK] = sum;
+  const double eb = exp(-b);
+  for (int k = kLocalK; k >= 1; --k) I[k - 1] = (b * I[k] + eb) / (double)k;
+  double pw = L, fact = 1.0;
+  for (int k = 0; k <= kLocalK; ++k) {
+    a[k] = pw * I[k] / fact;
+    pw *= -L;
+    fact *= (double)(k + 1);
+  }
+}
+
+extern "C" int wfl_local_taylor(double uc, double* a) {
+  if (!a || !(uc >= 0.0) || !isfinite(uc)) { wfl_set_error("wfl_local_taylor: need uc >= 0 and an output"); return WFL_ERR_ARG; }
+  local_taylor(uc, a);
+  return WFL_OK;
+}
+
+// Tuples of the FP32 Gaussian axis form whose loops (up to radius R) keep
+// every node within kLocalDmax of uc = c'^2 take the local node; |c'| is
+// kept at most 3 (uc <= 9).
+static const double kLocalCmax = 3.0;
+static double local_dmax(const DevTuple& d, double R) {
+  const double c = fabs((double)d.cx), s = d.s;
+  return s * R * (s * R + 2.0 * c);
+}
+static void set_local(DevTuple& d, double R) {
+  d.loc = 0;
+  const double c = fabs((double)d.cx);
+  if (!(c <= kLocalCmax) || !(local_dmax(d, R) <= (double)kLocalDmax)) return;
+  double a[kLocalK + 1];
+  const double uc = (double)d.cx * (double)d.cx;
+  local_taylor(uc, a);
+  d.loc = 1;
+  d.ta0 = a[0];
+  d.tec = exp2(-uc);
+  const float a1h = (float)a[1];
+  d.ta[0] = a1h;
+  d.ta[1] = (float)(a[1] - (double)a1h);
+  for (int k = 2; k <= kLocalK; ++k) d.ta[k] = (float)a[k];
 }
 
 // ------------------------------------------------------ geometry upload
@@ -667,6 +780,7 @@ struct wfl_plan_s {
   int sblob_f4 = 0, lds_splines = 0;
   int uni_splines = 0, sblob_uni_f4 = 0;   // uniform-grid spline rows (FP32 physical)
   int axis = 0;                            // axis-form records (see wfl_plan_axis_form)
+  bool has_local = false;                  // some tuples take the local Taylor node (lane-per-tuple kernels)
   double* d_gsum = nullptr;
   double* d_perloop = nullptr;
   double* d_wm1 = nullptr;
@@ -679,6 +793,7 @@ struct wfl_plan_s {
   hipStream_t gstream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;   // bracket the main kernel (timed runs)
   bool timed = false;
+  hipStream_t last = nullptr;              // stream of the latest run (wfl_plan_read waits for it)
 };
 
 extern "C" void wfl_plan_destroy(wfl_plan* p) {
@@ -855,12 +970,37 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
     fill_tuple(sg, geoms[gi], gi, b->tup_cx[t], b->tup_cy[t], sqrt(b->tup_T[t]), nl, p->tups[t]);
     if (b->precision == WFL_MIXED) p->tups[t].acc = tuple_needs_accurate(geoms[gi], p->tups[t], sg, rmax);
   }
-  // lane groups: tuples sorted by geometry and node kind, each such set
-  // padded to 64 lanes (one geometry and one node kind per wavefront)
+  // the axis form: mixed physical n = 2 batches whose tuples all have
+  // x_cm on the x axis and no lattice geometry
+  {
+    bool ax = b->precision == WFL_MIXED && b->mode == WFL_MODE_PHYSICAL && b->nsub == 2 && b->ntup > 0;
+    for (int g = 0; g < b->ngeom && ax; ++g) ax = !(b->geoms[g].lattice > 0.0);
+    for (int t = 0; t < b->ntup && ax; ++t) ax = p->tups[t].cy == 0.0f;
+    p->axis = ax ? 1 : 0;
+  }
+  // local Taylor node (Gaussian axis form): tuples whose nodes stay near the
+  // tube-relative centre for loops up to the 90th percentile of the
+  // ensemble's loop radii (the kernel checks each loop's own radius)
+  std::vector<double> ldmax(b->ntup, 0.0);
+  if (p->axis && p->prof_gauss) {
+    TRY(loops_info(L));
+    std::vector<double> rs = L->radius;
+    std::sort(rs.begin(), rs.end());
+    const double R = rs[std::min(rs.size() - 1, (size_t)(0.9 * (double)rs.size()))];
+    for (int t = 0; t < b->ntup; ++t) {
+      set_local(p->tups[t], R);
+      ldmax[t] = p->tups[t].loc ? local_dmax(p->tups[t], R) : 0.0;
+    }
+  }
+  // lane groups: tuples sorted by geometry and node kind (local Taylor
+  // node tuples by their node range), each such set padded to 64 lanes (one
+  // geometry and one node kind per wavefront)
   std::vector<int> order(b->ntup);
   for (int t = 0; t < b->ntup; ++t) order[t] = t;
-  auto key = [&](int t) { return 4LL * p->tups[t].geom + p->tups[t].acc; };
-  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return key(a) < key(c); });
+  auto key = [&](int t) { return 4LL * p->tups[t].geom + 2LL * p->tups[t].loc + p->tups[t].acc; };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+    return key(a) != key(c) ? key(a) < key(c) : ldmax[a] < ldmax[c];
+  });
   std::vector<int> lanes;
   std::vector<int> lg_nloops;
   for (size_t k = 0; k < order.size();) {
@@ -891,23 +1031,20 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
   // mapping (autotune heuristic: lane-per-tuple needs enough lane groups)
   // (mapping 3 measured no faster than 1 on the bench workload: it is opt-in or autotuned only)
   int mapping = b->mapping;
+  int nloc = 0;
+  for (const DevTuple& t : p->tups) nloc += t.loc;
+  p->has_local = nloc > 0;
   if (mapping == 0) {
     double util = b->ntup > 0 ? (double)b->ntup / (64.0 * std::max(nlg, 1)) : 0.0;
     mapping = (util >= 0.5 && p->ntasks >= 512) ? 1 : 2;
+    // the local Taylor node exists in the lane-per-tuple kernels only
+    if (p->has_local) mapping = 1;
   }
   if (mapping == 3 && !p->prof_gauss) mapping = 1;
   p->mapping = mapping;
 
   TRY(loops_records(L, b->precision));
-  // the axis form: mixed physical n = 2 batches whose tuples all have
-  // x_cm on the x axis and no lattice geometry
-  {
-    bool ax = b->precision == WFL_MIXED && b->mode == WFL_MODE_PHYSICAL && b->nsub == 2 && b->ntup > 0;
-    for (int g = 0; g < b->ngeom && ax; ++g) ax = !(b->geoms[g].lattice > 0.0);
-    for (int t = 0; t < b->ntup && ax; ++t) ax = p->tups[t].cy == 0.0f;
-    p->axis = ax ? 1 : 0;
-    if (p->axis) TRY(loops_records_axis(L));
-  }
+  if (p->axis) TRY(loops_records_axis(L));
   TRY(upload(&p->d_tups, p->tups.data(), p->tups.size()));
   TRY(upload(&p->d_geoms, geoms.data(), geoms.size()));
   TRY(upload(&p->d_lanes, lanes.data(), lanes.size()));
@@ -953,10 +1090,14 @@ static void fill_launch(const wfl_plan* p, WflLaunch& a, hipStream_t s) {
   a.sblob = p->d_sblob; a.sblob_f4 = p->sblob_f4; a.lds_splines = p->lds_splines;
   a.uni_splines = p->uni_splines; a.sblob_uni_f4 = p->sblob_uni_f4;
   a.axis = p->axis; a.rec_axis = p->L->reca;
+  a.linfo = p->L->info;
   a.prof_gauss = p->prof_gauss; a.spinor = p->b.spinor ? 1 : 0; a.mode = p->b.mode; a.nsub = p->b.nsub;
   a.split = p->mapping == 3;
   a.gsum = (p->mapping == 1 && p->cpg > 1) ? p->d_gpart : p->d_gsum;
-  a.perloop = p->d_perloop; a.wm1 = p->d_wm1;
+  // per-loop outputs only when asked for or when the split finalize reads
+  // them; W - 1 per loop only for the wave-per-loop group sums
+  a.perloop = (p->b.want_perloop || p->mapping == 3) ? p->d_perloop : nullptr;
+  a.wm1 = p->mapping == 2 ? p->d_wm1 : nullptr;
   a.nonfinite = p->d_nonfinite;
   a.wpb = p->wpb;
   a.persistent = p->persistent;
@@ -965,10 +1106,18 @@ static void fill_launch(const wfl_plan* p, WflLaunch& a, hipStream_t s) {
   a.stream = s;
 }
 
+// A stream under graph capture records work rather than running it.
+static bool hipStreamIsCapturing_safe(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (s == nullptr) return false;
+  return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
 extern "C" int wfl_plan_run(wfl_plan* p, void* stream) {
   if (!p) { wfl_set_error("wfl_plan_run: NULL plan"); return WFL_ERR_STATE; }
   if (p->b.ntup == 0) return WFL_OK;
   hipStream_t s = (hipStream_t)stream;
+  if (!hipStreamIsCapturing_safe(s)) p->last = s;
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, p->d_nonfinite, p->d_queue);
   HIP_TRY(hipGetLastError());
   WflLaunch a;
@@ -1046,6 +1195,9 @@ static int plan_apply(wfl_plan* p, int mapping, int persistent, int wpb, int cpg
     int rc = plan_set_tasks(p, cpg);
     if (rc) return rc;
   }
+  // buffers the configuration no longer reads
+  if (mapping != 3 && !p->b.want_perloop && p->d_perloop) { (void)hipFree(p->d_perloop); p->d_perloop = nullptr; }
+  if (mapping != 2 && p->d_wm1) { (void)hipFree(p->d_wm1); p->d_wm1 = nullptr; }
   p->mapping = mapping;
   p->persistent = persistent;
   p->wpb = wpb;
@@ -1098,31 +1250,40 @@ extern "C" int wfl_plan_autotune(wfl_plan* p, void* stream, int* chosen) {
         cand.push_back(TuneVal{1, pers, wpb, c});
         if (p->prof_gauss && perloop_bytes <= 24.0e9) cand.push_back(TuneVal{3, pers, wpb, c});
       }
-  if (p->b.ntup < 2048 || few) cand.push_back(TuneVal{2, 0, 4, 1});
+  // (wave per loop has no local Taylor node: it would change the numbers)
+  if ((p->b.ntup < 2048 || few) && !p->has_local) cand.push_back(TuneVal{2, 0, 4, 1});
   TuneVal best = cand[0];
   float best_ms = 1e30f;
+  int timed = 0;
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   int rc = WFL_OK;
-  for (const TuneVal& c : cand) {
-    rc = plan_apply(p, c.mapping, c.persistent, c.wpb, c.cpg);
-    if (rc) break;
+  // one untimed run first (first-touch costs are not charged to a candidate)
+  rc = plan_apply(p, cand[0].mapping, cand[0].persistent, cand[0].wpb, cand[0].cpg);
+  if (!rc) rc = wfl_plan_run(p, stream);
+  if (!rc && hipStreamSynchronize((hipStream_t)stream) != hipSuccess) rc = WFL_ERR_DEVICE;
+  for (size_t k = 0; k < cand.size() && !rc; ++k) {
+    const TuneVal& c = cand[k];
+    // a candidate that cannot be set up or launched (allocation, launch
+    // limits) is skipped, not fatal: the next one is measured
+    if (plan_apply(p, c.mapping, c.persistent, c.wpb, c.cpg) != WFL_OK) continue;
     float ms = 0.0f;
     (void)hipEventRecord(e0, (hipStream_t)stream);
-    rc = wfl_plan_run(p, stream);                 // the whole run, every kernel of it
-    if (rc) break;
+    if (wfl_plan_run(p, stream) != WFL_OK) continue;  // the whole run, every kernel of it
     (void)hipEventRecord(e1, (hipStream_t)stream);
     if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
       wfl_set_error("wfl_plan_autotune: event timing failed");
       rc = WFL_ERR_DEVICE;
       break;
     }
+    ++timed;
     if (ms < best_ms) { best_ms = ms; best = c; }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
+  if (!timed) { wfl_set_error("wfl_plan_autotune: no launch configuration could run"); return WFL_ERR_DEVICE; }
   rc = plan_apply(p, best.mapping, best.persistent, best.wpb, best.cpg);
   if (rc) return rc;
   rc = wfl_plan_run(p, stream);      // leave the plan's outputs from the chosen configuration
@@ -1168,18 +1329,25 @@ extern "C" int wfl_plan_graph_run(wfl_plan* p, void* stream) {
     HIP_TRY(hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0));
   }
   HIP_TRY(hipGraphLaunch(p->gexec, (hipStream_t)stream));
+  p->last = (hipStream_t)stream;
   return WFL_OK;
 }
 
 extern "C" int wfl_plan_read(wfl_plan* p, double* gsum, double* perloop) {
   if (!p) { wfl_set_error("wfl_plan_read: NULL plan"); return WFL_ERR_STATE; }
-  HIP_TRY(hipDeviceSynchronize());
-  if (gsum && p->b.ntup > 0)
-    HIP_TRY(hipMemcpy(gsum, p->d_gsum, sizeof(double) * (size_t)p->b.ntup * p->ngroups * 3, hipMemcpyDeviceToHost));
-  if (perloop) {
-    if (!p->d_perloop) { wfl_set_error("wfl_plan_read: per-loop outputs were not requested"); return WFL_ERR_STATE; }
-    HIP_TRY(hipMemcpy(perloop, p->d_perloop, sizeof(double) * (size_t)p->b.ntup * p->L->Nl * 6, hipMemcpyDeviceToHost));
+  if (perloop && !(p->b.want_perloop && p->d_perloop)) {
+    wfl_set_error("wfl_plan_read: per-loop outputs were not requested");
+    return WFL_ERR_STATE;
   }
+  // the copies are ordered after the latest run on its stream
+  hipStream_t s = p->last;
+  if (gsum && p->b.ntup > 0)
+    HIP_TRY(hipMemcpyAsync(gsum, p->d_gsum, sizeof(double) * (size_t)p->b.ntup * p->ngroups * 3,
+                           hipMemcpyDeviceToHost, s));
+  if (perloop)
+    HIP_TRY(hipMemcpyAsync(perloop, p->d_perloop, sizeof(double) * (size_t)p->b.ntup * p->L->Nl * 6,
+                           hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
   return WFL_OK;
 }
 
@@ -1193,6 +1361,12 @@ extern "C" int wfl_plan_accurate_tuples(const wfl_plan* p) {
 extern "C" int wfl_plan_uniform_splines(const wfl_plan* p) { return p ? p->uni_splines : 0; }
 
 extern "C" int wfl_plan_axis_form(const wfl_plan* p) { return p ? p->axis : 0; }
+extern "C" int wfl_plan_local_tuples(const wfl_plan* p) {
+  if (!p) return 0;
+  int n = 0;
+  for (const DevTuple& t : p->tups) n += t.loc ? 1 : 0;
+  return n;
+}
 extern "C" const double* wfl_plan_gsum_device(const wfl_plan* p) { return p ? p->d_gsum : nullptr; }
 extern "C" int wfl_plan_mapping(const wfl_plan* p) { return p ? p->mapping : 0; }
 

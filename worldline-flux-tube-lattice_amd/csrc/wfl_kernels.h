@@ -58,6 +58,40 @@ __device__ __forceinline__ int cls_of(int i) { return (i & 1) ? 0 : ((i & 3) == 
 // 11 VALU per point besides the nodes (point_eval: 14).
 constexpr int RECA = 8;
 
+// Per-loop constants of the local Taylor node (axis form): the class sums
+// Q_c = sum_{i in c} w_i x (e_i + 4 dw_i) in FP64 and the loop's largest
+// radius max |w_i|.
+struct LoopInfo { double q[3]; double r; };
+
+// Local Taylor node (FP32 Gaussian axis form, scaled units).  For a loop
+// whose nodes all lie within |u - uc| <= kLocalDmax of uc = c'^2 (c' the
+// tube-relative x_cm), g(u) = (1 - 2^-u)/u is written as
+//   g(u) = a0 + h(d),  h(d) = sum_{k=1..7} a_k d^k,  d = u - uc,
+// with a_k = g^(k)(uc)/k!, and the spin node as 2^-u = 2^-uc 2^-d.  Since
+// the cross products of a sub-loop's segments with a fixed point sum to
+// zero, sum_i (z x v_i) a0 = a0 s' sum_i w_i x v_i = a0 s' Q_c, which is
+// added in FP64 once per loop; the FP32 sums carry only (z x v) h, whose
+// terms have no cancellation.  The phase then needs no transcendental:
+// the gfx950 exp2/log2/rcp results carry small biases that vary smoothly
+// with the argument, and in sum_i (c' x v_i) g(u_i) (terms ~ c'/(s' |w|)
+// times the result) such a bias does not average out.  Truncation of the
+// series at k = 7 is below 4e-9 relative in h for |d| <= 0.5 at any uc
+// >= 0 (|g^(k)| <= ln2^(k+1)/(k+1)).
+constexpr float kLocalDmax = 0.5f;
+struct LocalC {
+  float a1h, a1l, a2, a3, a4, a5, a6, a7;   // a1 = a1h + a1l
+};
+
+__device__ __forceinline__ float local_h(float d, const LocalC& t) {
+  float D = fmaf(t.a7, d, t.a6);
+  D = fmaf(D, d, t.a5);
+  D = fmaf(D, d, t.a4);
+  D = fmaf(D, d, t.a3);
+  D = fmaf(D, d, t.a2);
+  const float lo = fmaf(D, d, t.a1l);
+  return fmaf(d, t.a1h, d * lo);
+}
+
 // Per-lane constants of one tuple in the kernel's type and units.
 template <class R> struct LaneC {
   R s;          // sqrt(T)
@@ -309,6 +343,54 @@ __device__ __forceinline__ void image_sums(const R* __restrict__ rl, int N, R cc
   }
 }
 
+// Class sums of one loop in the axis form with the local Taylor node (see
+// LocalC): per point d = u - c^2 from the small terms t = s w_x and the
+// y parts, t (t + 2c) + s^2 w_y^2 (and the same at the midpoint), and h(d)
+// for the phase.  The spin node 2^-u is 2^-uc - D with
+// D = uc h + d (a0 + h) (from 1 - 2^-u = u g(u)), so the FP32 sums carry
+// only D, which is small against 2^-u, and no exp2 (whose results just
+// below 1 are biased by about -0.12 ulp on gfx950).  The FP32 partial sums
+// are flushed into P and S every 32 points, S as 3 n 2^-uc minus the sum of
+// D (n = the block's points of the class).  The caller adds a0 s Q_c to P.
+template <int SPINOR>
+__device__ __forceinline__ void axis_local_sums(const float* __restrict__ rl, int N, float c, float s,
+                                                const LocalC& tc, float a0f, float ucf, double ec, double P[3],
+                                                double S[3]) {
+  const float c2 = c + c, m8 = 0.125f * s, a2 = s * s;
+  for (int i0 = 0; i0 < N; i0 += 32) {
+    float pz[3] = {0.0f, 0.0f, 0.0f}, ps[3] = {0.0f, 0.0f, 0.0f}, q[3] = {0.0f, 0.0f, 0.0f};
+    const int i1 = i0 + 32 < N ? i0 + 32 : N;
+    for (int i = i0; i < i1; i += 4) {
+      const float* r4 = rl + (size_t)i * RECA;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = (u & 1) ? 0 : (u == 0 ? 2 : 1);
+        const float* r = r4 + RECA * u;
+        const float t = s * r[0];
+        const float zx = t + c;
+        const float dA = fmaf(t, t + c2, a2 * r[1]);
+        const float tm = fmaf(m8, r[2], t);
+        const float dm = fmaf(tm, tm + c2, a2 * r[3]);
+        const float hA = local_h(dA, tc), hm = local_h(dm, tc);
+        pz[k] = fmaf(zx, fmaf(r[5], hA, r[4] * hm), pz[k]);
+        ps[k] = fmaf(r[7], hA, fmaf(r[6], hm, ps[k]));
+        if (SPINOR) {
+          const float DA = fmaf(dA, a0f + hA, ucf * hA);
+          const float Dm = fmaf(dm, a0f + hm, ucf * hm);
+          q[k] += fmaf(2.0f, Dm, DA);
+        }
+      }
+    }
+    const double nb = (double)(i1 - i0);
+    const double cnt[3] = {0.5 * nb, 0.25 * nb, 0.25 * nb};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      P[k] += fma(-(double)s, (double)ps[k], (double)pz[k]);
+      if (SPINOR) S[k] += fma(3.0 * ec, cnt[k], -(double)q[k]);
+    }
+  }
+}
+
 // The FP32 Gaussian node for image tube c (centre-relative offset c' in
 // kernel units), one choice for the whole wavefront (ballots), so every lane
 // runs the same code: the accurate node when some lane's loops are small
@@ -362,7 +444,7 @@ __device__ __forceinline__ void tuple_task(
     const float4* __restrict__ sblob,
     const Task* __restrict__ tasks, int nslots, int nsub_rt,
     double* __restrict__ gsum, double* __restrict__ perloop,
-    unsigned long long* __restrict__ nonfinite) {
+    unsigned long long* __restrict__ nonfinite, const LoopInfo* __restrict__ linfo) {
   const Task tk = tasks[task_id];
   const int lg = __builtin_amdgcn_readfirstlane(tk.lanegroup);
   const int loop0 = __builtin_amdgcn_readfirstlane(tk.loop0);
@@ -389,11 +471,43 @@ __device__ __forceinline__ void tuple_task(
   const bool lattice = MODE != M_AXIS && gm.lattice > 0.0;
   constexpr int RS = MODE == M_AXIS ? RECA : REC;   // record stride
 
+  // local Taylor node: every lane of the group has one (the host groups them)
+  constexpr bool kLocal = MODE == M_AXIS && PROF == P_GAUSSIAN && sizeof(R) == 4;
+  const bool loc_group = kLocal && linfo && __builtin_amdgcn_readfirstlane(tp.loc) != 0;
+
   double G[3] = {0.0, 0.0, 0.0};
   for (int l = loop0; l < loop0 + nloop; ++l) {
     const R* __restrict__ rl = (const R*)__builtin_assume_aligned(rec + (size_t)l * N * RS, 2 * sizeof(R));
     double P[3] = {0.0, 0.0, 0.0}, S[3] = {0.0, 0.0, 0.0};
-    loop_sums<R, PROF, SPINOR, MODE, NSUB>(rl, N, M, lattice, tp, gm, cx, cy, lat, lc, kind, gp, sp, nsub_rt, P, S);
+    bool done = false;
+    if constexpr (kLocal) {
+      if (loc_group) {
+        // this loop's nodes stay within kLocalDmax of c^2 for this lane's
+        // tuple (lanes that differ take the two paths one after the other)
+        const double rlw = linfo[l].r;
+        const float r = (float)rlw;
+        const float dmax = lc.s * r * fmaf(lc.s, r, 2.0f * fabsf((float)cx));
+        if (dmax <= kLocalDmax) {
+          // the coefficients are read here, per loop, so that they occupy
+          // registers only on this path (hoisted, they would hold 12 VGPRs
+          // through the other path and cost a wavefront per SIMD)
+          int tl = tid;
+          asm volatile("" : "+v"(tl));
+          const DevTuple* __restrict__ tq = tups + tl;
+          const LocalC tc = {tq->ta[0], tq->ta[1], tq->ta[2], tq->ta[3], tq->ta[4], tq->ta[5], tq->ta[6],
+                             tq->ta[7]};
+          axis_local_sums<SPINOR>((const float*)rl, N, (float)cx, (float)lc.s, tc, (float)tq->ta0,
+                                  (float)((double)cx * (double)cx), tq->tec, P, S);
+          const double a0s = tq->ta0 * (double)lc.s;
+          P[0] = fma(a0s, linfo[l].q[0], P[0]);
+          P[1] = fma(a0s, linfo[l].q[1], P[1]);
+          P[2] = fma(a0s, linfo[l].q[2], P[2]);
+          done = true;
+        }
+      }
+    }
+    if (!done)
+      loop_sums<R, PROF, SPINOR, MODE, NSUB>(rl, N, M, lattice, tp, gm, cx, cy, lat, lc, kind, gp, sp, nsub_rt, P, S);
     double sc[3], fe[3];
     finalize_loop(P, S, gm.F, s_phase, tp.sd * tp.sd, spin_scale, n, N, SPINOR, sc, fe);
     if (!SPLIT) {
@@ -432,7 +546,8 @@ __device__ __forceinline__ int next_task(int* __restrict__ queue, int lane) {
     const float4* __restrict__ sblob, int sblob_f4,                                          \
     const Task* __restrict__ tasks, int ntasks, int nslots, int nsub_rt,                     \
     double* __restrict__ gsum, double* __restrict__ perloop,                                 \
-    unsigned long long* __restrict__ nonfinite, int wpb, int persistent, int* __restrict__ queue
+    unsigned long long* __restrict__ nonfinite, int wpb, int persistent, int* __restrict__ queue, \
+    const LoopInfo* __restrict__ linfo
 
 template <class R, int PROF, int SPINOR, int MODE, int NSUB, int SPLIT>
 __device__ __forceinline__ void k_tuple_body(WFL_K_TUPLE_PARAMS) {
@@ -442,7 +557,7 @@ __device__ __forceinline__ void k_tuple_body(WFL_K_TUPLE_PARAMS) {
   int task_id = persistent ? next_task(queue, lane) : (int)blockIdx.x * wpb + wv;
   while (task_id < ntasks) {
     tuple_task<R, PROF, SPINOR, MODE, NSUB, SPLIT>(task_id, lane, rec, N, Nl, tups, geoms, lanes, sb,
-                                                   tasks, nslots, nsub_rt, gsum, perloop, nonfinite);
+                                                   tasks, nslots, nsub_rt, gsum, perloop, nonfinite, linfo);
     if (!persistent) break;
     task_id = next_task(queue, lane);
   }
@@ -451,7 +566,8 @@ __device__ __forceinline__ void k_tuple_body(WFL_K_TUPLE_PARAMS) {
 template <class R, int PROF, int SPINOR, int MODE, int NSUB, int SPLIT>
 __global__ __launch_bounds__(256) void k_tuple(WFL_K_TUPLE_PARAMS) {
   k_tuple_body<R, PROF, SPINOR, MODE, NSUB, SPLIT>(rec, N, Nl, tups, geoms, lanes, sblob, sblob_f4, tasks, ntasks,
-                                                   nslots, nsub_rt, gsum, perloop, nonfinite, wpb, persistent, queue);
+                                                   nslots, nsub_rt, gsum, perloop, nonfinite, wpb, persistent, queue,
+                                                   linfo);
 }
 
 // ------------------------------------------------------- wave per loop
@@ -479,18 +595,14 @@ __device__ __forceinline__ double shfl_xor_d(double v, int m) {
   return __shfl_xor(v, m, 64);
 }
 
+// One (tuple, loop) pair w = t Nl + l of the wave-per-loop mapping.
 template <class R, int PROF, int SPINOR, int MODE, int NSUB>
-__global__ __launch_bounds__(256) void k_loop(
-    const R* __restrict__ rec, int N, int Nl, const DevTuple* __restrict__ tups,
-    const DevGeom* __restrict__ geoms, int ntup,
-    const float4* __restrict__ sblob, int sblob_f4, int nsub_rt,
+__device__ __forceinline__ void loop_pair(
+    long long w, const R* __restrict__ rec, int N, int Nl, const DevTuple* __restrict__ tups,
+    const DevGeom* __restrict__ geoms, const float4* __restrict__ sb, int nsub_rt,
     double* __restrict__ perloop, double* __restrict__ wm1,
     float4* __restrict__ ws_scal, float4* __restrict__ ws_ferm,
-    unsigned long long* __restrict__ nonfinite, int wpb) {
-  const float4* sb = stage_splines<PROF>(sblob, sblob_f4);   // before any wavefront leaves
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const long long w = (long long)blockIdx.x * wpb + wv;
-  if (w >= (long long)ntup * Nl) return;
+    unsigned long long* __restrict__ nonfinite) {
   const int t = (int)(w / Nl), l = (int)(w % Nl);
   const int lane = threadIdx.x & 63;
   const DevTuple tp = tups[t];
@@ -553,6 +665,24 @@ __global__ __launch_bounds__(256) void k_loop(
   }
 }
 
+// Wavefront w of the grid takes the pairs w, w + W, w + 2W, ... (W = the
+// grid's wavefronts), so a capped grid covers any ntup x Nl.
+template <class R, int PROF, int SPINOR, int MODE, int NSUB>
+__global__ __launch_bounds__(256) void k_loop(
+    const R* __restrict__ rec, int N, int Nl, const DevTuple* __restrict__ tups,
+    const DevGeom* __restrict__ geoms, int ntup,
+    const float4* __restrict__ sblob, int sblob_f4, int nsub_rt,
+    double* __restrict__ perloop, double* __restrict__ wm1,
+    float4* __restrict__ ws_scal, float4* __restrict__ ws_ferm,
+    unsigned long long* __restrict__ nonfinite, int wpb) {
+  const float4* sb = stage_splines<PROF>(sblob, sblob_f4);   // before any wavefront leaves
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long total = (long long)ntup * Nl, stride = (long long)gridDim.x * wpb;
+  for (long long w = (long long)blockIdx.x * wpb + wv; w < total; w += stride)
+    loop_pair<R, PROF, SPINOR, MODE, NSUB>(w, rec, N, Nl, tups, geoms, sb, nsub_rt, perloop, wm1, ws_scal, ws_ferm,
+                                           nonfinite);
+}
+
 }  // namespace wfl
 
 // Launch interface of one precision's kernel instantiations.
@@ -560,6 +690,7 @@ struct WflLaunch {
   const void* rec; int N, Nl;
   const float* rec_axis;                      // axis-form records (FP32, lane-per-tuple kernels)
   int axis;                                   // 1: every tuple on the x axis, no lattice, n = 2
+  const wfl::LoopInfo* linfo;                 // per-loop constants of the local Taylor node (axis form)
   const wfl::DevTuple* tups; const wfl::DevGeom* geoms; int ntup;
   const int* lanes; const wfl::Task* tasks; int ntasks; int nslots;
   const float4* sblob; int sblob_f4;          // spline blob and its size in float4

@@ -58,6 +58,12 @@ struct DevTuple {
   float cx, cy, s;              // FP32 kernel units
   int acc;                      // 1: the loops can reach a tube core with the accurate node (groups lanes)
   double cxd, cyd, sd, padd;    // effective values (FP64 kernels)
+  // Local Taylor node of the FP32 Gaussian axis form (loc = 1; see
+  // wfl_kernels.h): g(uc) and 2^-uc in FP64 at uc = cx^2, and the FP32
+  // coefficients a1 (split in two floats), a2 ... a7.
+  double ta0, tec;
+  float ta[8];
+  int loc, pad_loc;
 };
 
 // ---------------------------------------------------------------- math

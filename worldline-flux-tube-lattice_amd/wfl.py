@@ -80,7 +80,8 @@ EXPORTS = [
     "wfl_loops_destroy", "wfl_gen_loops", "wfl_read_text_loops",
     "wfl_plan_create", "wfl_plan_run", "wfl_plan_run_timed", "wfl_plan_autotune", "wfl_plan_config",
     "wfl_plan_set_config", "wfl_plan_graph_run", "wfl_plan_read", "wfl_plan_ngroups", "wfl_plan_accurate_tuples",
-    "wfl_plan_uniform_splines", "wfl_autotune_nsub", "wfl_plan_axis_form",
+    "wfl_plan_uniform_splines", "wfl_autotune_nsub", "wfl_plan_axis_form", "wfl_plan_local_tuples",
+    "wfl_local_taylor",
     "wfl_plan_gsum_device", "wfl_plan_nonfinite", "wfl_plan_mapping", "wfl_plan_destroy",
     "wfl_eval_batch", "wfl_group_ev_minus1", "wfl_spline_build", "wfl_set_profile",
     "wfl_set_numerics", "wfl_ev_reset", "wfl_ev_launches", "wfl_logsimpson", "wfl_effective_T",
@@ -125,6 +126,8 @@ def _load():
         "wfl_plan_accurate_tuples": (I, [P]),
         "wfl_plan_uniform_splines": (I, [P]),
         "wfl_plan_axis_form": (I, [P]),
+        "wfl_plan_local_tuples": (I, [P]),
+        "wfl_local_taylor": (I, [D, P]),
         "wfl_autotune_nsub": (I, [P, P, C.c_double, I, I, I, P, P]),
         "wfl_plan_gsum_device": (P, [P]),
         "wfl_plan_nonfinite": (I, [P, P]),
@@ -351,6 +354,7 @@ class Plan:
         self.accurate_tuples = lib.wfl_plan_accurate_tuples(self.h)
         self.uniform_splines = lib.wfl_plan_uniform_splines(self.h)
         self.axis_form = lib.wfl_plan_axis_form(self.h)
+        self.local_tuples = lib.wfl_plan_local_tuples(self.h)
         self.mapping = lib.wfl_plan_mapping(self.h)
         self.ntup = len(batch.tup_geom)
 

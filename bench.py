@@ -324,23 +324,33 @@ def load_loops(wfl, cfg):
 
 # ------------------------------------------------------------------ runs
 def interaction_energy(wfl, cfg, gsum):
-    """Per-tube action of the lattice cell minus the isolated tube's action.
-    Cell: midpoint rule over the 32 x 32 x_cm grid, (l^2/1024) sum L_eff.
-    Isolated: 2 pi int rho L_eff(rho) drho (Simpson on the 49-point rho grid)."""
-    L = leff_table(wfl, cfg, gsum, 1)
-    iso = L[-1]                                  # [rho, group]
-    rho = cfg["rhos"][-1]
-    h = rho[1] - rho[0]
-    w = np.full(len(rho), 2.0)
-    w[1::2] = 4.0
-    w[0] = w[-1] = 1.0
-    s_iso_groups = h / 3 * (w[:, None] * 2 * np.pi * rho[:, None] * iso).sum(0)
+    """Interaction energy per tube (interaction.sage:10-25) through the
+    library: the cell action (midpoint rule over the 32 x 32 x_cm grid,
+    wfl_cell_action_groups), the isolated tube's action (Simpson in rho on
+    its 49-point grid, wfl_radial_action_groups), and E_int = mean of
+    E_cell - E_iso with E = -action and the script's spread
+    (wfl_interaction_energy)."""
+    tg = cfg["tup"][0]
+    T = cfg["Ts"][0]
+    nT = len(T)
+    ng = gsum.shape[1]
+    counts = np.array([min(GROUP, cfg["Nl"] - q * GROUP) for q in range(ng)], np.int32)
+    sp = cfg["spinor"]
+    gi = len(cfg["geoms"]) - 1                   # the isolated tube
+    rho = cfg["rhos"][gi]
+    iso = wfl.radial_action_groups(wfl.Geometry(**cfg["geoms"][gi]), sp, T, rho,
+                                   gsum[tg == gi].reshape(len(rho), nT, ng, 3), counts)
     res = []
     for i, l in enumerate(cfg["lattices"]):
-        e = L[i].sum(0) * (l * l / 1024.0) - s_iso_groups
-        res.append({"l": l, "E_int": float(e.mean()),
-                    "E_int_sem": float(e.std(ddof=1) / np.sqrt(len(e))) if len(e) > 1 else None})
-    return {"isolated_action": float(s_iso_groups.mean()), "per_tube": res}
+        cx, cy = cfg["rhos"][i], cfg["cys"][i]
+        w = np.full(len(cx), l * l / len(cx))
+        cell = wfl.cell_action_groups(wfl.Geometry(**cfg["geoms"][i]), sp, T, cx, cy, w,
+                                      gsum[tg == i].reshape(len(cx), nT, ng, 3), counts)
+        m, spread, sem = wfl.interaction_energy(cell, iso) if ng > 1 else (float(-(cell - iso).mean()), None, None)
+        res.append({"l": l, "E_int": m, "E_int_spread": spread, "E_int_sem": sem,
+                    "cell_action": float(cell.mean())})
+    return {"isolated_action": float(iso.mean()), "energy_convention": "E = -action (interaction.sage:13-16)",
+            "per_tube": res}
 
 
 def run_config(wfl, args, mode, cfg):

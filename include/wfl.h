@@ -266,6 +266,26 @@ int wfl_eaction(const wfl_geometry *g, wfl_loops *L, int fermion, int mode, int 
 /* Output level of the rho-integral drivers (EffAct.c's verbosity; default 1). */
 int wfl_set_verbosity(int v);
 
+/* ---- lattice physics outputs (analysis-scripts/interaction.sage:10-25) ----
+ * Per-group action of one tube of a lattice geometry: sum_c w_c L_eff(x_c)
+ * over ncell x_cm points of the unit cell (midpoint rule on an n x n grid:
+ * w_c = l^2/n^2).  gsum [ncell][nT][ngroups][3] holds the group sums of the
+ * points' tuples in T order (wfl_leff_groups for each point). */
+int wfl_cell_action_groups(const wfl_geometry *g, int fermion, int nT, const double *T, int ncell, const double *cx,
+                           const double *cy, const double *w, const double *gsum, const int *counts, int ngroups,
+                           int policy, double *action);
+/* Per-group action of an isolated tube: 2 pi int rho L_eff(rho) drho over
+ * nrho uniformly spaced rho >= 0 (composite Simpson, 3/8 rule on the last
+ * three intervals for an odd interval count).  gsum [nrho][nT][ngroups][3]. */
+int wfl_radial_action_groups(const wfl_geometry *g, int fermion, int nT, const double *T, int nrho, const double *rho,
+                             const double *gsum, const int *counts, int ngroups, int policy, double *action);
+/* Interaction energy per tube from per-group actions of the lattice tube
+ * and of the isolated tube on the same loop groups, as interaction.sage
+ * forms it: E = -action, out[0] = mean of E_cell - E_iso, out[1] = the
+ * script's error sqrt(var E_cell + var E_iso - 2 cov) (the spread of one
+ * group's difference), out[2] = out[1]/sqrt(ngroups) (standard error). */
+int wfl_interaction_energy(int ngroups, const double *cell_action, const double *iso_action, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -239,3 +239,49 @@ def leff_groups(geom, fermion, T, cx, cy, gsum, counts, policy=1):
         I = logsimpson(np.asarray(T, np.float64), f)
         out[q] = I / (4 * np.pi) if fermion else I / (-2 * np.pi)
     return out
+
+
+def simpson_uniform(h, f):
+    """Composite Simpson on uniformly spaced values; the 3/8 rule on the last
+    three intervals when the interval count is odd (trapezoid for two
+    values)."""
+    f = np.asarray(f, np.float64)
+    m = len(f) - 1
+    if m < 1:
+        return 0.0
+    if m == 1:
+        return 0.5 * h * (f[0] + f[1])
+    ms = m if m % 2 == 0 else m - 3
+    s = sum(h / 3.0 * (f[i] + 4.0 * f[i + 1] + f[i + 2]) for i in range(0, ms, 2))
+    if ms != m:
+        s += 3.0 * h / 8.0 * (f[ms] + 3.0 * f[ms + 1] + 3.0 * f[ms + 2] + f[ms + 3])
+    return s
+
+
+def cell_action_groups(geom, fermion, T, cx, cy, w, gsum, counts, policy=1):
+    """Per-group action of a lattice tube: sum_c w_c L_eff(x_c), gsum
+    [ncell, nT, ngroups, 3]."""
+    return sum(wc * leff_groups(geom, fermion, T, a, b, gsum[c], counts, policy)
+               for c, (a, b, wc) in enumerate(zip(cx, cy, w)))
+
+
+def radial_action_groups(geom, fermion, T, rho, gsum, counts, policy=1):
+    """Per-group action of an isolated tube: 2 pi int rho L_eff drho,
+    composite Simpson on the uniform rho grid; gsum [nrho, nT, ngroups, 3]."""
+    L = np.array([leff_groups(geom, fermion, T, r, 0.0, gsum[k], counts, policy) for k, r in enumerate(rho)])
+    f = 2.0 * np.pi * np.asarray(rho)[:, None] * L
+    h = (rho[-1] - rho[0]) / (len(rho) - 1)
+    return np.array([simpson_uniform(h, f[:, q]) for q in range(f.shape[1])])
+
+
+def interaction_energy(cell_action, iso_action):
+    """analysis-scripts/interaction.sage:10-25: energies E = -action, the
+    mean of E_cell - E_iso over groups and the script's error
+    sqrt(var(E_cell) + var(E_iso) - 2 cov(E_cell, E_iso)) with R's sample
+    (n - 1) var and cov; also that error / sqrt(n)."""
+    ec = -np.asarray(cell_action, np.float64)
+    ei = -np.asarray(iso_action, np.float64)
+    n = len(ec)
+    var = np.var(ec, ddof=1) + np.var(ei, ddof=1) - 2.0 * np.cov(ec, ei, ddof=1)[0, 1]
+    err = float(np.sqrt(max(var, 0.0)))
+    return float(np.mean(ec - ei)), err, err / np.sqrt(n)

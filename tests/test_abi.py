@@ -305,3 +305,62 @@ def test_local_taylor_coefficients(wfl):
             worst = max(worst, abs(float((series - exact) / exact)))
         assert worst < 4e-9, (uc, worst)
     assert wfl.lib.wfl_local_taylor(-1.0, np.zeros(8).ctypes.data) == -1
+
+
+def test_interaction_energy_statistics(wfl, orc):
+    """wfl_interaction_energy against interaction.sage's formula restated in
+    the oracle (mean of E_cell - E_iso with E = -action, and
+    sqrt(var + var - 2 cov)), and the formula's fixed points: equal inputs
+    give 0 and 0, a constant shift of the cell actions gives minus that
+    shift with zero spread."""
+    rng = np.random.default_rng(5)
+    iso = rng.normal(-0.3, 0.05, 40)
+    cell = iso + rng.normal(0.01, 0.002, 40)
+    got = wfl.interaction_energy(cell, iso)
+    ref = orc.interaction_energy(cell, iso)
+    assert np.allclose(got, ref, rtol=1e-12, atol=0)
+    # the difference's spread is the std of the per-group differences
+    assert got[1] == pytest.approx(np.std(cell - iso, ddof=1), rel=1e-9)
+    assert wfl.interaction_energy(iso, iso) == (0.0, 0.0, 0.0)
+    m, e, s = wfl.interaction_energy(iso + 0.25, iso)
+    assert m == pytest.approx(-0.25, rel=1e-12) and e < 1e-7 and s < 1e-7
+
+
+def test_radial_and_cell_action_quadrature(wfl, orc):
+    """The lattice output quadratures against the oracle restatement on
+    oracle group sums (a Gaussian tube, 8 loops x 64 points, 2 groups), and
+    the radial Simpson weights exact for cubic integrands (even and odd
+    interval counts)."""
+    x, y = orc.gen_loops(9, 8, 64)
+    T = np.logspace(-2, 1, 7)
+    counts = np.array([4, 4], np.int32)
+    geom = wfl.Geometry(kind=wfl.GAUSSIAN, F=1.0, l2=1.0)
+    og = orc.Geom(orc.GAUSSIAN, F=1.0, l2=1.0)
+    for nrho in (5, 6):
+        rho = np.linspace(0.0, 2.0, nrho)
+        tg = np.zeros(nrho * len(T), np.int32)
+        gs, _ = orc.eval_batch([og], orc.MODE_PHYSICAL, 2, 1, x, y, 4, tg, np.repeat(rho, len(T)), 0 * tg,
+                               np.tile(T, nrho), perloop=False)
+        gs = gs.reshape(nrho, len(T), 2, 3)
+        got = wfl.radial_action_groups(geom, 1, T, rho, gs, counts)
+        ref = orc.radial_action_groups(og, 1, T, rho, gs, counts)
+        assert np.allclose(got, ref, rtol=1e-12, atol=1e-300)
+    # lattice cell: 2 x 2 midpoint grid of a lattice geometry
+    lat = wfl.Geometry(kind=wfl.GAUSSIAN, F=1.0, l2=1.0, lattice=2.0, images=1)
+    olat = orc.Geom(orc.GAUSSIAN, F=1.0, l2=1.0, lattice=2.0, images=1)
+    cx = np.array([0.5, 1.5, 0.5, 1.5]); cy = np.array([0.5, 0.5, 1.5, 1.5]); w = np.full(4, 1.0)
+    tg = np.zeros(4 * len(T), np.int32)
+    gs, _ = orc.eval_batch([olat], orc.MODE_PHYSICAL, 2, 0, x, y, 4, tg, np.repeat(cx, len(T)), np.repeat(cy, len(T)),
+                           np.tile(T, 4), perloop=False)
+    gs = gs.reshape(4, len(T), 2, 3)
+    got = wfl.cell_action_groups(lat, 0, T, cx, cy, w, gs, counts)
+    ref = orc.cell_action_groups(olat, 0, T, cx, cy, w, gs, counts)
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-300)
+    # Simpson (odd count) and Simpson + 3/8 (even count) are exact for cubics
+    for n in (9, 10):
+        h = 0.3
+        f = np.array([(k * h) ** 3 - 2 * (k * h) + 1 for k in range(n)])
+        b = (n - 1) * h
+        assert orc.simpson_uniform(h, f) == pytest.approx(b ** 4 / 4 - b ** 2 + b, rel=1e-13)
+    with pytest.raises(wfl.WflError):
+        wfl.radial_action_groups(geom, 1, T, np.array([0.0, 0.5, 1.7]), np.zeros((3, len(T), 2, 3)), counts)

@@ -341,3 +341,100 @@ done:
   free(fixed); free(Bv); free(Te); free(acc); free(gsum);
   return rc;
 }
+
+/* ------------------------------------------------------------------ */
+/* Lattice physics outputs (analysis-scripts/interaction.sage:10-25)     */
+/* ------------------------------------------------------------------ */
+/* Per-group action of one lattice tube: the L_eff of each x_cm point of
+ * the unit cell weighted by w_c (the midpoint rule over an n x n grid has
+ * w_c = l^2 / n^2).  gsum [ncell][nT][ngroups][3]: the group sums of the
+ * cell points' tuples, each in T order. */
+int wfl_cell_action_groups(const wfl_geometry *g, int fermion, int nT, const double *T, int ncell, const double *cx,
+                           const double *cy, const double *w, const double *gsum, const int *counts, int ngroups,
+                           int policy, double *action) {
+  if (!g || !T || !cx || !cy || !w || !gsum || !counts || !action || nT < 2 || ncell < 1 || ngroups < 1) {
+    wfl_set_error("wfl_cell_action_groups: bad arguments");
+    return WFL_ERR_ARG;
+  }
+  double *leff = (double *)malloc(sizeof(double) * (size_t)ngroups);
+  if (!leff) { wfl_set_error("wfl_cell_action_groups: malloc"); return WFL_ERR_NOMEM; }
+  for (int q = 0; q < ngroups; ++q) action[q] = 0.0;
+  for (int c = 0; c < ncell; ++c) {
+    int rc = wfl_leff_groups(g, fermion, nT, T, cx[c], cy[c], gsum + (size_t)c * nT * ngroups * 3, counts, ngroups,
+                             policy, leff);
+    if (rc) { free(leff); return rc; }
+    for (int q = 0; q < ngroups; ++q) action[q] += w[c] * leff[q];
+  }
+  free(leff);
+  return WFL_OK;
+}
+
+/* Composite Simpson on n uniformly spaced values (the 3/8 rule on the last
+ * three intervals when the interval count is odd; trapezoid for n = 2). */
+static double simpson_uniform(double h, const double *f, int n) {
+  const int m = n - 1;
+  if (m < 1) return 0.0;
+  if (m == 1) return 0.5 * h * (f[0] + f[1]);
+  const int ms = (m % 2 == 0) ? m : m - 3;
+  double s = 0.0;
+  for (int i = 0; i + 2 <= ms; i += 2) s += h / 3.0 * (f[i] + 4.0 * f[i + 1] + f[i + 2]);
+  if (ms != m) s += 3.0 * h / 8.0 * (f[ms] + 3.0 * f[ms + 1] + 3.0 * f[ms + 2] + f[ms + 3]);
+  return s;
+}
+
+/* Per-group action of an isolated tube, 2 pi int rho L_eff(rho) drho over
+ * nrho uniformly spaced rho (composite Simpson).  gsum [nrho][nT][ngroups][3]. */
+int wfl_radial_action_groups(const wfl_geometry *g, int fermion, int nT, const double *T, int nrho, const double *rho,
+                             const double *gsum, const int *counts, int ngroups, int policy, double *action) {
+  if (!g || !T || !rho || !gsum || !counts || !action || nT < 2 || nrho < 2 || ngroups < 1) {
+    wfl_set_error("wfl_radial_action_groups: bad arguments");
+    return WFL_ERR_ARG;
+  }
+  const double h = (rho[nrho - 1] - rho[0]) / (double)(nrho - 1);
+  for (int r = 1; r < nrho; ++r)
+    if (!(fabs(rho[r] - rho[0] - h * r) <= 1e-9 * (fabs(rho[nrho - 1]) + h))) {
+      wfl_set_error("wfl_radial_action_groups: rho must be uniformly spaced");
+      return WFL_ERR_ARG;
+    }
+  double *f = (double *)malloc(sizeof(double) * ((size_t)nrho * ngroups + ngroups));
+  if (!f) { wfl_set_error("wfl_radial_action_groups: malloc"); return WFL_ERR_NOMEM; }
+  double *leff = f + (size_t)nrho * ngroups;
+  for (int r = 0; r < nrho; ++r) {
+    int rc = wfl_leff_groups(g, fermion, nT, T, rho[r], 0.0, gsum + (size_t)r * nT * ngroups * 3, counts, ngroups,
+                             policy, leff);
+    if (rc) { free(f); return rc; }
+    for (int q = 0; q < ngroups; ++q) f[(size_t)q * nrho + r] = 2.0 * WFL_PI * rho[r] * leff[q];
+  }
+  for (int q = 0; q < ngroups; ++q) action[q] = simpson_uniform(h, f + (size_t)q * nrho, nrho);
+  free(f);
+  return WFL_OK;
+}
+
+/* Interaction energy per tube as interaction.sage:10-25 forms it from
+ * per-group results: energies E = -action (the script negates the action
+ * column, :13-16), the group-wise difference E_cell - E_iso, its mean, and
+ * the script's error sqrt(var(E_cell) + var(E_iso) - 2 cov(E_cell, E_iso))
+ * (sample variance and covariance, R's var/cov, :23-24), which is the
+ * spread of one group's difference; out = {mean, that spread, spread /
+ * sqrt(ngroups) (the standard error of the mean)}.  Energies in units of
+ * the electron mass (the script's toMeV = 510.9989 converts to keV). */
+int wfl_interaction_energy(int ngroups, const double *cell_action, const double *iso_action, double *out) {
+  if (ngroups < 2 || !cell_action || !iso_action || !out) {
+    wfl_set_error("wfl_interaction_energy: need >= 2 groups and buffers");
+    return WFL_ERR_ARG;
+  }
+  double mc = 0.0, mi = 0.0;
+  for (int q = 0; q < ngroups; ++q) { mc += -cell_action[q]; mi += -iso_action[q]; }
+  mc /= ngroups; mi /= ngroups;
+  double vc = 0.0, vi = 0.0, cv = 0.0;
+  for (int q = 0; q < ngroups; ++q) {
+    const double dc = -cell_action[q] - mc, di = -iso_action[q] - mi;
+    vc += dc * dc; vi += di * di; cv += dc * di;
+  }
+  const double n1 = (double)(ngroups - 1);
+  const double var = vc / n1 + vi / n1 - 2.0 * cv / n1;
+  out[0] = mc - mi;
+  out[1] = sqrt(var > 0.0 ? var : 0.0);
+  out[2] = out[1] / sqrt((double)ngroups);
+  return WFL_OK;
+}

@@ -90,6 +90,7 @@ EXPORTS = [
     "EV", "EVMean", "Tfunc", "integrateT", "getwl", "Exact", "MagField", "smTscal", "smTferm",
     "rhogrand", "testvalrho", "transformRho", "integrateRhotoa", "integrateRho", "PeriodicCA", "FixFluxCA",
     "ClassAction", "EAction", "wfl_eaction", "wfl_set_verbosity",
+    "wfl_cell_action_groups", "wfl_radial_action_groups", "wfl_interaction_energy",
     "wfl_loops_write", "wfl_loops_read_header", "wfl_loops_read", "wfl_write_text_loops",
 ]
 
@@ -158,6 +159,9 @@ def _load():
         "EAction": (P, [I, P]),
         "wfl_eaction": (I, [P, P, I, I, I, I, I, I, I, P, P, P, P]),
         "wfl_set_verbosity": (I, [I]),
+        "wfl_cell_action_groups": (I, [P, I, I, P, I, P, P, P, P, P, I, I, P]),
+        "wfl_radial_action_groups": (I, [P, I, I, P, I, P, P, P, I, I, P]),
+        "wfl_interaction_energy": (I, [I, P, P, P]),
         "getwl": (I, [P, C.c_char_p, I, I, I]),
         "Exact": (D, [D, D, I]),
         "MagField": (D, [D, P]),
@@ -457,6 +461,46 @@ def leff_groups(geom: Geometry, fermion, T, cx, cy, gsum, counts, policy=1):
     check(lib.wfl_leff_groups(C.byref(g), int(fermion), len(T_eff), _ptr(T_eff), float(cx), float(cy),
                               _ptr(gsum), _ptr(counts), ng, int(policy), _ptr(out)))
     return out
+
+
+def cell_action_groups(geom: Geometry, fermion, T, cx, cy, w, gsum, counts, policy=1):
+    """Per-group action of one lattice tube: sum_c w_c L_eff(x_c); gsum
+    [ncell, nT, ngroups, 3] (the cell points' tuples in T order)."""
+    T = np.ascontiguousarray(T, np.float64)
+    cx, cy, w = (np.ascontiguousarray(a, np.float64) for a in (cx, cy, w))
+    gsum = np.ascontiguousarray(gsum, np.float64)
+    counts = np.ascontiguousarray(counts, np.int32)
+    ng = gsum.shape[2]
+    out = np.zeros(ng, np.float64)
+    g = geom.c()
+    check(lib.wfl_cell_action_groups(C.byref(g), int(fermion), len(T), _ptr(T), len(cx), _ptr(cx), _ptr(cy), _ptr(w),
+                                     _ptr(gsum), _ptr(counts), ng, int(policy), _ptr(out)))
+    return out
+
+
+def radial_action_groups(geom: Geometry, fermion, T, rho, gsum, counts, policy=1):
+    """Per-group action of an isolated tube, 2 pi int rho L_eff drho on a
+    uniform rho grid; gsum [nrho, nT, ngroups, 3]."""
+    T = np.ascontiguousarray(T, np.float64)
+    rho = np.ascontiguousarray(rho, np.float64)
+    gsum = np.ascontiguousarray(gsum, np.float64)
+    counts = np.ascontiguousarray(counts, np.int32)
+    ng = gsum.shape[2]
+    out = np.zeros(ng, np.float64)
+    g = geom.c()
+    check(lib.wfl_radial_action_groups(C.byref(g), int(fermion), len(T), _ptr(T), len(rho), _ptr(rho), _ptr(gsum),
+                                       _ptr(counts), ng, int(policy), _ptr(out)))
+    return out
+
+
+def interaction_energy(cell_action, iso_action):
+    """(mean E_cell - E_iso with E = -action, the spread sqrt(var + var -
+    2 cov) of interaction.sage, its standard error)."""
+    a = np.ascontiguousarray(cell_action, np.float64)
+    b = np.ascontiguousarray(iso_action, np.float64)
+    out = np.zeros(3, np.float64)
+    check(lib.wfl_interaction_energy(len(a), _ptr(a), _ptr(b), _ptr(out)))
+    return tuple(float(v) for v in out)
 
 
 def eaction(geom: Geometry, loops: "Loops", fermion, mode=MODE_PHYSICAL, precision=MIXED, nsub=2, group=128,

@@ -245,9 +245,25 @@ def oracle_geoms(cfg):
     return orc, geoms
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform string."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
+
+
 def cpu_baseline(cfg, x, y, mode_id, seconds=12.0, full=False):
     """FP64 C oracle (OpenMP over loops) on a bounded sample of the workload,
-    or on the whole of it with full=True (then also returns its group sums)."""
+    or on the whole of it with full=True.  Returns (baseline record, oracle
+    group sums, sample) where sample = (tuple indices, loops): with
+    full=True all tuples and loops in groups of GROUP, otherwise the timed
+    sample's tuples on its first loops as one group."""
     orc, geoms = oracle_geoms(cfg)
     cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
     tg, tcx, tcy, tT = cfg["tup"]
@@ -256,8 +272,9 @@ def cpu_baseline(cfg, x, y, mode_id, seconds=12.0, full=False):
         gs, _ = orc.eval_batch(geoms, mode_id, cfg["nsub"], cfg["spinor"], x, y, GROUP, tg, tcx, tcy, tT,
                                perloop=False, nthreads=cores)
         dt = time.perf_counter() - t0
-        return {"value": point_evals(cfg) / dt, "unit": "point-evals/s", "cores": cores, "kind": "port",
-                "sample": f"the whole workload, FP64 C oracle, {dt:.1f} s"}, gs
+        return {"value": point_evals(cfg) / dt, "unit": "point-evals/s", "cores": cores, "cpu_model": cpu_model(),
+                "kind": "port", "sample": f"the whole workload, FP64 C oracle, {dt:.1f} s"}, gs, \
+            (np.arange(len(tg)), x.shape[0])
     nl = min(64, x.shape[0])
     stride = max(1, len(tg) // 64)
     idx = np.arange(0, len(tg), stride)
@@ -269,13 +286,30 @@ def cpu_baseline(cfg, x, y, mode_id, seconds=12.0, full=False):
     stride2 = max(1, stride // scale)
     idx = np.arange(0, len(tg), stride2)
     t0 = time.perf_counter()
-    orc.eval_batch(geoms, mode_id, cfg["nsub"], cfg["spinor"], x[:nl], y[:nl], nl, tg[idx], tcx[idx], tcy[idx],
-                   tT[idx], perloop=False, nthreads=cores)
+    gs, _ = orc.eval_batch(geoms, mode_id, cfg["nsub"], cfg["spinor"], x[:nl], y[:nl], nl, tg[idx], tcx[idx], tcy[idx],
+                           tT[idx], perloop=False, nthreads=cores)
     dt = time.perf_counter() - t0
     pe = nl * cfg["N"] * len(idx)
-    return {"value": pe / dt, "unit": "point-evals/s", "cores": cores, "kind": "port",
+    return {"value": pe / dt, "unit": "point-evals/s", "cores": cores, "cpu_model": cpu_model(), "kind": "port",
             "sample": f"first {nl} loops x {cfg['N']} points x every {stride2}th of {len(tg)} tuples "
-                      f"({len(idx)} tuples), FP64 C oracle, {dt:.1f} s"}, None
+                      f"({len(idx)} tuples), FP64 C oracle, {dt:.1f} s"}, gs, (idx, nl)
+
+
+def fp64_vs_oracle(wfl, cfg, x, y, mode_id, gs_oracle, sample):
+    """The FP64 GPU engine on exactly the oracle's sample (its tuples, its
+    first loops in its groups): largest group-sum difference over the
+    largest oracle group sum."""
+    idx, nl = sample
+    group = GROUP if nl == x.shape[0] else nl
+    L = wfl.Loops(np.ascontiguousarray(x[:nl]), np.ascontiguousarray(y[:nl]))
+    try:
+        tg, tcx, tcy, tT = (a[idx] for a in cfg["tup"])
+        b = wfl.Batch(make_geoms(wfl, cfg), tg, tcx, tcy, tT, mode=mode_id, precision=wfl.FP64, nsub=cfg["nsub"],
+                      spinor=cfg["spinor"], group=group, splines=cfg.get("splines", []))
+        gs, _ = wfl.eval_batch(b, L)
+    finally:
+        L.close()
+    return float(np.max(np.abs(gs - gs_oracle)) / max(np.max(np.abs(gs_oracle)), 1e-300))
 
 
 class Runner:
@@ -406,11 +440,13 @@ def run_config(wfl, args, mode, cfg):
     if not args.no_cpu:
         log("cpu baseline...")
         full = args.config == "c1"
-        cb, gs_cpu = cpu_baseline(cfg, x, y, mode, full=full)
+        cb, gs_cpu, sample = cpu_baseline(cfg, x, y, mode, full=full)
         out["cpu_baseline"] = cb
-        if gs_cpu is not None and gs_d is not None:
+        if full and gs_d is not None:
             out["fp64_gpu_vs_cpu_oracle_max_rel_err"] = float(np.max(np.abs(gs_d - gs_cpu)) /
                                                              max(np.max(np.abs(gs_cpu)), 1e-300))
+        elif not args.no_fp64:
+            out["fp64_gpu_vs_cpu_oracle_max_rel_err"] = fp64_vs_oracle(wfl, cfg, x, y, mode, gs_cpu, sample)
     print(json.dumps(out))
 
 
@@ -668,11 +704,15 @@ def bench_c2(wfl, args, mode):
     result["roofline"] = {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32_VECTOR / 1e12,
                           "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_VECTOR, "traffic": None,
                           "kernel": f"k_tuple<float,GAUSSIAN,spinor,{'axis' if axis else 'physical'},n=2>",
+                          "kernel_time_source": "HIP events around the kernel on its launch stream",
                           "kernel_ms": k_mixed * 1e3, "flops_per_point_eval": fl}
-    tr = os.path.join(ROOT, "profiles", f"traffic_{cfg['name']}.json")
-    if os.path.exists(tr):
-        with open(tr) as fh:
+    # HBM bytes per launch come from rocprofv3 FETCH_SIZE / WRITE_SIZE passes of
+    # this same command (counters cannot be read from inside the run)
+    tr = os.path.join("profiles", f"traffic_{cfg['name']}.json")
+    if os.path.exists(os.path.join(ROOT, tr)):
+        with open(os.path.join(ROOT, tr)) as fh:
             result["roofline"]["traffic"] = json.load(fh).get("hbm_bytes_per_launch")
+        result["roofline"]["traffic_source"] = tr
     result["nonfinite_loops"] = nonfinite
     result["accurate_node_tuples"] = accurate
     result["local_node_tuples"] = local
@@ -700,7 +740,11 @@ def bench_c2(wfl, args, mode):
     loops.close()
     if not args.no_cpu:
         log("cpu baseline...")
-        result["cpu_baseline"], _ = cpu_baseline(cfg, x, y, mode)
+        result["cpu_baseline"], gs_cpu, sample = cpu_baseline(cfg, x, y, mode)
+        if not args.no_fp64:
+            # the FP64 engine (the ratio's denominator) against the oracle on the baseline's sample
+            result["fp64_gpu_vs_cpu_oracle_max_rel_err"] = fp64_vs_oracle(wfl, cfg, x, y, mode, gs_cpu, sample)
+            result["fp64_gpu_vs_cpu_oracle_sample"] = result["cpu_baseline"]["sample"]
     print(json.dumps(result))
 
 

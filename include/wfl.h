@@ -251,6 +251,19 @@ double wfl_effective_T(const wfl_geometry *g, double T);
  * /(4 pi) spinor. */
 int wfl_leff_groups(const wfl_geometry *g, int fermion, int nT, const double *T, double cx, double cy,
                     const double *gsum, const int *counts, int ngroups, int policy, double *leff);
+/* The same with a choice of the N -> infinity estimate of <W> - 1 per
+ * group: estimator 0 the reference's Richardson combination
+ * 8/3 W_N - 2 W_{N/2} + 1/3 W_{N/4} (Integrand.c:400, as wfl_leff_groups);
+ * 1 the two-level extrapolation 2 W_{N/2} - W_{N/4}.  The reference's W_N
+ * is cos(x/2 + (y + z)/4), a blend of the sub-loop phases, whose
+ * discretization error is not half of W_{N/2}'s, so its weights leave a
+ * 1/N term (about +6.5 % in a constant-field scalar L_eff at N = 1024);
+ * estimator 1 cancels it.  (An extension: no counterpart in the reference.) */
+int wfl_leff_groups_est(const wfl_geometry *g, int fermion, int nT, const double *T, double cx, double cy,
+                        const double *gsum, const int *counts, int ngroups, int policy, int estimator,
+                        double *leff);
+/* <W> - 1 of one group by estimator (0 reference, 1 two-level). */
+double wfl_group_ev_minus1_est(const double gsum3[3], int count, int estimator);
 
 /* Effective action of an isolated analytic tube with the batched engine:
  * EAction's rho quadrature (EffAct.c:265-306, integrateRho with Npoints
@@ -263,8 +276,20 @@ int wfl_leff_groups(const wfl_geometry *g, int fermion, int nT, const double *T,
  * Not for spline or lattice geometries (WFL_ERR_ARG). */
 int wfl_eaction(const wfl_geometry *g, wfl_loops *L, int fermion, int mode, int precision, int nsub, int group,
                 int Npoints, int nT, double *groups_out, double *result, double *se, double *classical);
-/* Output level of the rho-integral drivers (EffAct.c's verbosity; default 1). */
+/* Output level of the drivers (the reference's verbosity; default 1): at 2
+ * and above Tfunc prints its branch traces and the WLvconstExpression:
+ * line (rho^2, T, Exact, smTscal, mean and spread of the integrand over the
+ * groups; Integrand.c:255-313), at 3 the EV= line. */
 int wfl_set_verbosity(int v);
+
+/* Floating-point exception checks in EVMean (Integrand.c:318-342, where
+ * they are always on and process-wide): 0 off (default); 1 report (the
+ * four flags divide-by-zero, invalid, overflow, underflow are cleared on
+ * entry and tested on exit; a raised flag prints the reference's message
+ * and is counted); 2 strict (the reference's traps and SIGFPE handler:
+ * message and exit status 1). */
+int wfl_set_fp_traps(int mode);
+long long wfl_fp_trap_count(void);
 
 /* ---- lattice physics outputs (analysis-scripts/interaction.sage:10-25) ----
  * Per-group action of one tube of a lattice geometry: sum_c w_c L_eff(x_c)

@@ -215,7 +215,7 @@ def field_at(geom, cx, cy):
     return B, r.value
 
 
-def leff_groups(geom, fermion, T, cx, cy, gsum, counts, policy=1):
+def leff_groups(geom, fermion, T, cx, cy, gsum, counts, policy=1, estimator=0):
     """Oracle L_eff per group: Tfunc's branch policy at each node's effective
     parameters, composite Simpson in ln T over the nominal nodes, /(-2pi)
     scalar or /(4pi) spinor (Tint.c:69-72).  The field is the image-window
@@ -229,7 +229,10 @@ def leff_groups(geom, fermion, T, cx, cy, gsum, counts, policy=1):
             t = rt * rt
             B, rho = field_at(geom, ex, ey)
             g3 = gsum[k, q]
-            evm1 = (8.0 / 3.0 * g3[0] - 2.0 * g3[1] + 1.0 / 3.0 * g3[2]) / counts[q]
+            if estimator == 1:   # two-level extrapolation 2 W_{N/2} - W_{N/4}
+                evm1 = (2.0 * g3[1] - g3[2]) / counts[q]
+            else:                # Integrand.c:400
+                evm1 = (8.0 / 3.0 * g3[0] - 2.0 * g3[1] + 1.0 / 3.0 * g3[2]) / counts[q]
             b = C.c_int(-1)
             v = lib.orc_igrand_B(geom.ref, fermion, t, B, rho, 1.0 + evm1, C.byref(b))
             if b.value == 3 or not policy:
@@ -285,3 +288,72 @@ def interaction_energy(cell_action, iso_action):
     var = np.var(ec, ddof=1) + np.var(ei, ddof=1) - 2.0 * np.cov(ec, ei, ddof=1)[0, 1]
     err = float(np.sqrt(max(var, 0.0)))
     return float(np.mean(ec - ei)), err, err / np.sqrt(n)
+
+
+def _rule_nodes(n):
+    """integrateT's / integrateRho's rule on t in (0, 1] (Tint.c:100-175,
+    EffAct.c:137-199): per interval the right end twice and the midpoint,
+    h/6 (2 f(t_(i+1)) + 4 f(t_(i+1/2))); returns (t, weight)."""
+    h = 1.0 / n
+    t, w = [], []
+    for i in range(n):
+        t += [h * (i + 1.0), h * (i + 0.5)]
+        w += [h / 6.0 * 2.0, h / 6.0 * 4.0]
+    return np.array(t), np.array(w)
+
+
+def eaction_groups(geom, x, y, fermion, mode, nsub, group, npoints, nT, lcf=False, nthreads=0):
+    """Per-group effective action of an isolated tube as EffAct.c:265-306
+    computes it: integrateRho(npoints) over rho = (1 - t)/t with
+    rhogrand's factor rho and 1/t^2, integrateT(nT) over T = (1 - t)/t with
+    1/t^2, Tfunc's branch policy (Integrand.c:195-316) at each node's
+    effective parameters (Geom.tuple), the worldline estimate from the
+    oracle's group sums, and the /(4 pi) (spinor) or /(-2 pi) (scalar)
+    normalisation.  lcf=True takes the constant-field closed form Exact at
+    every node (T > 0) instead: the locally-constant-field action on the
+    same rules."""
+    tr, wr = _rule_nodes(npoints)
+    tt, wt = _rule_nodes(nT)
+    rho = (1.0 - tr) / tr
+    T = (1.0 - tt) / tt
+    Nl = x.shape[0]
+    ng = (Nl + group - 1) // group
+    counts = np.array([min(group, Nl - q * group) for q in range(ng)])
+    fixed = np.zeros((len(rho), len(T)))
+    need = []
+    for i, r in enumerate(rho):
+        for k, t in enumerate(T):
+            if r == 0.0:
+                continue
+            ex, ey, rt = geom.tuple(r, 0.0, t)
+            te = rt * rt
+            B = lib.orc_magfield(geom.ref, ex * ex)
+            if lcf:
+                fixed[i, k] = exact(te, B, fermion) if te > 0 else 0.0
+                continue
+            v, b = igrand(geom, fermion, te, abs(ex), ex * ex, 1.0)
+            if b == 3 and t > 0.0:
+                need.append((i, k, te, B))
+            else:
+                fixed[i, k] = 0.0 if b == 3 else v
+    gs = None
+    if need:
+        gs, _ = eval_batch([geom], mode, nsub, fermion, x, y, group, np.zeros(len(need), np.int32),
+                           np.array([rho[i] for i, _, _, _ in need]), np.zeros(len(need)),
+                           np.array([T[k] for _, k, _, _ in need]), perloop=False, nthreads=nthreads)
+    out = np.zeros(ng)
+    for q in range(ng):
+        f = fixed.copy()
+        for n, (i, k, te, B) in enumerate(need):
+            g3 = gs[n, q]
+            evm1 = (8.0 / 3.0 * g3[0] - 2.0 * g3[1] + 1.0 / 3.0 * g3[2]) / counts[q]
+            TB = 0.30282212 * te * B
+            f[i, k] = np.exp(-te) / te ** 3 * (evm1 - TB * TB / 3.0 if fermion else evm1 + TB * TB / 6.0)
+        f[np.isinf(f)] = 0.0
+        I = f @ (wt / (tt * tt))
+        out[q] = np.sum(wr * rho / (tr * tr) * I)
+    return out / (4.0 * np.pi) if fermion else out / (-2.0 * np.pi)
+
+
+def exact(T, B, fermion):
+    return lib.orc_exact(T, B, fermion)

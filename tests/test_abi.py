@@ -364,3 +364,115 @@ def test_radial_and_cell_action_quadrature(wfl, orc):
         assert orc.simpson_uniform(h, f) == pytest.approx(b ** 4 / 4 - b ** 2 + b, rel=1e-13)
     with pytest.raises(wfl.WflError):
         wfl.radial_action_groups(geom, 1, T, np.array([0.0, 0.5, 1.7]), np.zeros((3, len(T), 2, 3)), counts)
+
+
+def _c_stdout(capfd, fn):
+    """Run fn and return what the C library printed (stdio flushed)."""
+    libc = C.CDLL(None)
+    libc.fflush(None)
+    capfd.readouterr()
+    try:
+        fn()
+    finally:
+        libc.fflush(None)
+    return capfd.readouterr().out
+
+
+def test_tfunc_verbosity_lines(wfl, orc, capfd):
+    """Tfunc at verbosity 2 prints the reference's branch traces and its
+    WLvconstExpression: line (Integrand.c:255-313): rho^2, T, Exact,
+    smTscal, and the mean and meanigrand spread of the integrand over the
+    groups (here every group takes the same analytic branch, so the spread
+    is 0).  Small-T and large-T branches need no device."""
+    wfl.set_profile(wfl.Geometry(kind=wfl.GAUSSIAN, F=1.0, l2=1.0))
+    p = wfl.Wparams()
+    p.Nl, p.Nppl, p.F, p.l2, p.ng, p.fermion = 256, 64, 1.0, 1.0, 2, 1
+    p.xcm.x, p.xcm.y = 0.5, 0.0
+    order = np.arange(256, dtype=np.int32)
+    og = orc.Geom(orc.GAUSSIAN, F=1.0, l2=1.0, quantize="float")
+    B = orc.lib.orc_magfield(og.ref, 0.25)
+    wfl.lib.wfl_set_verbosity(2)
+    try:
+        for T, tag, ref_ig in ((0.005, "using small T expression:", orc.lib.orc_smTferm(B, 0.005)),
+                               (60.0, "using large T expression", -np.exp(-60.0) / 180.0 * (0.30282212 * B) ** 2)):
+            out = _c_stdout(capfd, lambda: wfl.lib.Tfunc(T, C.byref(p), order.ctypes.data))
+            lines = out.strip().splitlines()
+            assert sum(l.startswith(tag) for l in lines) == 2, out
+            wl = [l for l in lines if l.startswith("WLvconstExpression:")]
+            assert len(wl) == 1, out
+            v = [float(t) for t in wl[0].split()[1:]]
+            assert v[0] == pytest.approx(0.25, rel=1e-6) and v[1] == pytest.approx(T, rel=1e-6)
+            assert v[2] == pytest.approx(orc.lib.orc_exact(T, B, 1), rel=1e-5, abs=1e-300)
+            assert v[3] == pytest.approx(orc.lib.orc_smTscal(B, T), rel=1e-5, abs=1e-300)
+            assert v[4] == pytest.approx(ref_ig, rel=1e-5) and v[5] == 0.0
+        wfl.lib.wfl_set_verbosity(1)
+        out = _c_stdout(capfd, lambda: wfl.lib.Tfunc(0.005, C.byref(p), order.ctypes.data))
+        assert "WLvconstExpression" not in out and "using" not in out
+    finally:
+        wfl.lib.wfl_set_verbosity(1)
+
+
+def test_fp_trap_modes(wfl, capfd):
+    """EVMean's floating-point checks (Integrand.c:318-342): off by default
+    (an invalid operation passes silently), mode 1 reports and counts, mode
+    2 is the reference's trap and exit (status 1), run in a child process."""
+    ws = np.zeros((4, 4), np.float32)
+    wf = np.zeros((4, 4), np.float32)
+    ws[1, 0] = np.inf                      # cos(inf): invalid
+    WL = np.arange(4, dtype=np.int32)
+    ev = C.c_double()
+    run = lambda: wfl.lib.EVMean(C.byref(ev), ws.ctypes.data, wf.ctypes.data, 4, WL.ctypes.data, 1.0, 0)
+    out = _c_stdout(capfd, run)
+    assert "Floating point" not in out
+    n0 = wfl.lib.wfl_fp_trap_count()
+    wfl.check(wfl.lib.wfl_set_fp_traps(1))
+    try:
+        out = _c_stdout(capfd, run)
+        assert "Error: Floating point signal detected." in out
+        assert wfl.lib.wfl_fp_trap_count() == n0 + 1
+        ws[1, 0] = 0.0
+        out = _c_stdout(capfd, run)          # clean inputs raise nothing
+        assert "Floating point" not in out and wfl.lib.wfl_fp_trap_count() == n0 + 1
+    finally:
+        wfl.check(wfl.lib.wfl_set_fp_traps(0))
+    assert wfl.lib.wfl_set_fp_traps(3) == -1
+    import subprocess
+    import sys
+    code = (
+        "import ctypes as C, numpy as np, sys\n"
+        f"sys.path.insert(0, {ROOT!r})\n"
+        "import __graft_entry__ as ge\n"
+        "w = ge._import_pkg()\n"
+        "ws = np.zeros((4, 4), np.float32); ws[1, 0] = np.inf\n"
+        "wf = np.zeros((4, 4), np.float32); WL = np.arange(4, dtype=np.int32); ev = C.c_double()\n"
+        "w.lib.wfl_set_fp_traps(2)\n"
+        "w.lib.EVMean(C.byref(ev), ws.ctypes.data, wf.ctypes.data, 4, WL.ctypes.data, 1.0, 0)\n"
+        "print('not reached')\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1, (r.returncode, r.stdout, r.stderr)
+    assert "Error: Floating point signal detected." in r.stdout and "not reached" not in r.stdout
+
+
+def test_two_level_estimator(wfl, orc):
+    """The two-level N -> infinity estimate 2 W_{N/2} - W_{N/4} removes a
+    discretization error a/N_k exactly (W_{N/2} = W + 2a/N, W_{N/4} =
+    W + 4a/N), where the reference's 8/3, -2, 1/3 on (W_N, W_{N/2}, W_{N/4})
+    does only if W_N's error is a/N; L_eff with either estimator equals the
+    oracle's."""
+    W, a, N, cnt = -0.0123, 0.37, 1024.0, 128
+    g3 = np.array([0.0, W + 2 * a / N, W + 4 * a / N]) * cnt
+    assert wfl.lib.wfl_group_ev_minus1_est(g3.ctypes.data, cnt, 1) == pytest.approx(W, rel=1e-13)
+    g3[0] = (W + a / N) * cnt
+    assert wfl.lib.wfl_group_ev_minus1_est(g3.ctypes.data, cnt, 0) == pytest.approx(W, rel=1e-12)
+    g3[0] = (W + 1.7 * a / N) * cnt           # a blend whose error is not a/N
+    assert abs(wfl.lib.wfl_group_ev_minus1_est(g3.ctypes.data, cnt, 0) - W) > 1e-4
+    x, y = orc.gen_loops(9, 8, 64)
+    T = np.logspace(-2, 1, 7)
+    og = orc.Geom(orc.GAUSSIAN, F=1.0, l2=1.0)
+    gs, _ = orc.eval_batch([og], orc.MODE_PHYSICAL, 2, 0, x, y, 4, np.zeros(len(T), np.int32), np.full(len(T), 0.3),
+                           np.zeros(len(T)), T, perloop=False)
+    geom = wfl.Geometry(kind=wfl.GAUSSIAN, F=1.0, l2=1.0)
+    for est in (0, 1):
+        got = wfl.leff_groups(geom, 0, T, 0.3, 0.0, gs, np.array([4, 4]), 0, est)
+        ref = orc.leff_groups(og, 0, T, 0.3, 0.0, gs, np.array([4, 4]), 0, est)
+        assert np.allclose(got, ref, rtol=1e-12)

@@ -315,3 +315,36 @@ def test_fp32_profile_identities():
             ref = 1 - (om * mp.exp(-1 / om) + mp.ei(-1 / om)) / (2 * q2)
             integral = mp.quad(lambda s: mp.exp(-1 / s), [om, 1])
             assert abs(phi0 + integral / (2 * q2) - ref) < 1e-25
+
+
+def test_eaction_rules_converge_to_lcf_quadrature(orc):
+    """The oracle's restatement of EAction's rho and T rules (integrateRho,
+    integrateT: the right end twice and the midpoint per interval, rho and
+    T = (1 - t)/t with 1/t^2, rhogrand's factor rho, /(4 pi)), run on the
+    constant-field closed form Exact at every node, converges to an
+    independent adaptive quadrature of the locally-constant-field action
+    int rho drho int dT Exact(T, B(rho))/(4 pi) (scipy; x coth x - 1 - x^2/3
+    by its series below x = 0.05) for a wide Gaussian tube (F = 50,
+    lambda^2 = 100, the testT.c:10-20 tube): within 1e-6 at 80 x 160
+    intervals, and the reference's 20 x 40 within 1e-2."""
+    from scipy import integrate
+    g = orc.Geom(orc.GAUSSIAN, F=50.0, l2=100.0, quantize=False)
+    e = 0.30282212
+
+    def gfun(xv):
+        if abs(xv) < 0.05:
+            x2 = xv * xv
+            return x2 * x2 * (-1 / 45 + x2 * (2 / 945 - x2 / 4725))
+        return xv / np.tanh(xv) - 1 - xv * xv / 3
+
+    def inner(rho):
+        B = orc.lib.orc_magfield(g.ref, rho * rho)
+        return integrate.quad(lambda T: np.exp(-T) / T ** 3 * gfun(e * T * B), 0, 60, limit=400, epsabs=0,
+                              epsrel=1e-12)[0]
+
+    ref = integrate.quad(lambda r: r * inner(r), 0, 60, limit=400, epsabs=0, epsrel=1e-11)[0] / (4 * np.pi)
+    x = np.zeros((2, 8), np.float32)
+    fine = orc.eaction_groups(g, x, x, 1, 1, 2, 2, 80, 160, lcf=True)[0]
+    coarse = orc.eaction_groups(g, x, x, 1, 1, 2, 2, 20, 40, lcf=True)[0]
+    assert abs(fine - ref) <= 1e-6 * abs(ref)
+    assert abs(coarse - ref) <= 1e-2 * abs(ref)

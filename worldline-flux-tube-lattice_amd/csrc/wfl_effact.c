@@ -29,6 +29,8 @@ int wfl_set_verbosity(int v) {
   return WFL_OK;
 }
 
+int wfl_internal_verbosity(void) { return g_verbosity; }
+
 /* ------------------------------------------------------------------ */
 /* Drop-in drivers (reference names)                                    */
 /* ------------------------------------------------------------------ */

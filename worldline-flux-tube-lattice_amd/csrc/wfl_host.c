@@ -14,7 +14,10 @@
  *   wfl_spline_build  periodic.c:45-110 (clamped cubic spline, zero slopes)
  *   wfl_logsimpson, wfl_leff_groups   T-integral on log-spaced grids
  */
+#define _GNU_SOURCE   /* feenableexcept (glibc) for the strict FP-trap mode */
+#include <fenv.h>
 #include <math.h>
+#include <signal.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -151,6 +154,56 @@ double EV(double T, void *p, int *WLlist) {
   return ev;
 }
 
+/* ------------------------------------------------------------------ */
+/* Floating-point exception checks (Integrand.c:318-342)                */
+/* ------------------------------------------------------------------ */
+/* The reference enables traps for division by zero, invalid, overflow
+ * and underflow on every EVMean call and exits from a SIGFPE handler
+ * (process-wide state, Integrand.c:338-342).  Here the checks are a
+ * library mode, off by default:
+ *   0  off;
+ *   1  report: EVMean clears the four flags on entry and tests them on
+ *      exit; a raised flag prints the reference's message and is counted
+ *      (wfl_fp_trap_count), the call completes;
+ *   2  strict: the reference's behaviour, feenableexcept for the four and a
+ *      SIGFPE handler that prints the message and exits with status 1. */
+#define WFL_FP_FLAGS (FE_DIVBYZERO | FE_INVALID | FE_OVERFLOW | FE_UNDERFLOW)
+static int g_fp_mode = 0;
+static long long g_fp_count = 0;
+
+static void fp_signal_handler(int sig) {
+  (void)sig;
+  printf("Error: Floating point signal detected.\n");
+  fflush(stdout);
+  exit(1);
+}
+
+int wfl_set_fp_traps(int mode) {
+  if (mode < 0 || mode > 2) { wfl_set_error("wfl_set_fp_traps: mode must be 0, 1 or 2"); return WFL_ERR_ARG; }
+  if (g_fp_mode == 2 && mode != 2) fedisableexcept(WFL_FP_FLAGS);
+  g_fp_mode = mode;
+  return WFL_OK;
+}
+
+long long wfl_fp_trap_count(void) { return g_fp_count; }
+
+static void fp_check_begin(void) {
+  if (g_fp_mode == 1) feclearexcept(WFL_FP_FLAGS);
+  if (g_fp_mode == 2) {
+    feclearexcept(WFL_FP_FLAGS);
+    feenableexcept(WFL_FP_FLAGS);
+    signal(SIGFPE, fp_signal_handler);
+  }
+}
+
+static void fp_check_end(void) {
+  if (g_fp_mode == 1 && fetestexcept(WFL_FP_FLAGS)) {
+    printf("Error: Floating point signal detected.\n");
+    ++g_fp_count;
+    feclearexcept(WFL_FP_FLAGS);
+  }
+}
+
 /* abs() of a double in Integrand.c:371 converts to int first; on x86-64 an
  * out-of-range conversion yields INT_MIN and the guard stays off. */
 static int guard600(double v) {
@@ -159,6 +212,7 @@ static int guard600(double v) {
 }
 
 void EVMean(double *EVout, float4 *Wsscal_h, float4 *Wsferm_h, int n, int *WL, double T, int fermion) {
+  fp_check_begin();
   double e1 = 0.0, e2 = 0.0, e4 = 0.0;
   for (int i = 0; i < n; ++i) {
     int w = WL[i];
@@ -185,11 +239,26 @@ void EVMean(double *EVout, float4 *Wsscal_h, float4 *Wsferm_h, int n, int *WL, d
   }
   e4 /= n; e2 /= n; e1 /= n;
   *EVout = 8.0 / 3.0 * e1 - 2.0 * e2 + 1.0 / 3.0 * e4;
+  fp_check_end();
 }
 
 double wfl_group_ev_minus1(const double g3[3], int count) {
   if (count <= 0) return 0.0;
   return (8.0 / 3.0 * g3[0] - 2.0 * g3[1] + 1.0 / 3.0 * g3[2]) / (double)count;
+}
+
+/* <W> - 1 of one group by estimator: 0 the reference's Richardson
+ * combination (Integrand.c:400); 1 the two-level extrapolation
+ * 2 W_{N/2} - W_{N/4} of the sub-loop estimates, whose discretization errors
+ * scale as 1/(N/2) and 1/(N/4), so the 1/N term cancels.  (The reference's
+ * W_N is cos(x/2 + (y + z)/4), a blend of the same sub-loop phases rather
+ * than the N-point loop's, so its error is not half of W_{N/2}'s and the
+ * 8/3, -2, 1/3 weights leave a 1/N term: about +6.5 % in the scalar L_eff
+ * of a constant field at N = 1024.) */
+double wfl_group_ev_minus1_est(const double g3[3], int count, int estimator) {
+  if (count <= 0) return 0.0;
+  if (estimator == 1) return (2.0 * g3[1] - g3[2]) / (double)count;
+  return wfl_group_ev_minus1(g3, count);
 }
 
 /* ------------------------------------------------------------------ */
@@ -356,9 +425,19 @@ double *Tfunc(double T, void *p, int *order) {
    * host buffers already hold them, so the other groups only average: the
    * same numbers and the same final buffer contents. */
   int filled = 0;
+  const int verb = wfl_internal_verbosity();
+  const double sT = smallT_geom(&g, rho);
   for (int i = 0; i < params->ng; ++i) {
     int branch;
     double v = igrand_geom(&g, params->fermion, T, rho, rho2, 0.0, &branch);
+    /* branch traces of Integrand.c:255-288 (the large-T line prints the
+     * small-T scale third; the reference's format names a third value it
+     * does not pass) */
+    if (verb >= 2) {
+      if (branch == 0) printf("using small T expression: %f %f %f\n", rho2, T, sT);
+      else if (branch == 1) printf("using const. field expression %f %f %f\n", rho2, T, sT);
+      else if (branch == 2) printf("using large T expression %f %f %f\n", rho2, T, sT);
+    }
     if (branch == 3) {
       for (int j = 0; j < groupsize; ++j) WLlist[j] = order[j + i * groupsize];
       double ev = 0.0;
@@ -368,9 +447,21 @@ double *Tfunc(double T, void *p, int *order) {
         ev = EV(T, p, WLlist);
         filled = !isnan(ev);
       }
+      if (verb >= 3) printf("EV= %e Exact = %e\n", ev, T / sinh(T));   /* Integrand.c:294-295 */
       v = igrand_geom(&g, params->fermion, T, rho, rho2, ev - 1.0, &branch);
     }
     igrand[i] = v;
+  }
+  if (T > 0.0 && verb >= 2) {
+    /* Integrand.c:307-313 with meanigrand (Integrand.c:174-192): the mean
+     * over groups and sqrt(sum of squared deviations)/ng */
+    double m = 0.0, e = 0.0;
+    for (int i = 0; i < params->ng; ++i) m += igrand[i];
+    m /= params->ng;
+    for (int i = 0; i < params->ng; ++i) e += (igrand[i] - m) * (igrand[i] - m);
+    e = sqrt(e) / params->ng;
+    const double B = magfield_geom(&g, rho2);
+    printf("WLvconstExpression: %e %e %e %e %e %e\n", rho2, T, Exact(T, B, params->fermion), smTscal(B, T), m, e);
   }
   free(WLlist);
   return igrand;
@@ -447,7 +538,16 @@ double wfl_logsimpson(const double *T, const double *f, int n) {
  * small-/large-T branches, policy 0 the worldline estimate at every node. */
 int wfl_leff_groups(const wfl_geometry *g, int fermion, int nT, const double *T, double cx, double cy,
                     const double *gsum, const int *counts, int ngroups, int policy, double *leff) {
-  if (!g || !T || !gsum || !counts || !leff || nT < 2 || ngroups < 1) { wfl_set_error("wfl_leff_groups: bad arguments"); return WFL_ERR_ARG; }
+  return wfl_leff_groups_est(g, fermion, nT, T, cx, cy, gsum, counts, ngroups, policy, 0, leff);
+}
+
+int wfl_leff_groups_est(const wfl_geometry *g, int fermion, int nT, const double *T, double cx, double cy,
+                        const double *gsum, const int *counts, int ngroups, int policy, int estimator,
+                        double *leff) {
+  if (!g || !T || !gsum || !counts || !leff || nT < 2 || ngroups < 1 || estimator < 0 || estimator > 1) {
+    wfl_set_error("wfl_leff_groups: bad arguments");
+    return WFL_ERR_ARG;
+  }
   double *f = (double *)malloc(sizeof(double) * 5 * (size_t)nT);
   if (!f) { wfl_set_error("wfl_leff_groups: malloc"); return WFL_ERR_NOMEM; }
   double *Te = f + nT, *B = f + 2 * nT, *sT = f + 3 * nT, *rr = f + 4 * nT;
@@ -465,7 +565,7 @@ int wfl_leff_groups(const wfl_geometry *g, int fermion, int nT, const double *T,
   }
   for (int q = 0; q < ngroups; ++q) {
     for (int k = 0; k < nT; ++k) {
-      double evm1 = wfl_group_ev_minus1(gsum + ((size_t)k * ngroups + q) * 3, counts[q]);
+      double evm1 = wfl_group_ev_minus1_est(gsum + ((size_t)k * ngroups + q) * 3, counts[q], estimator);
       double t = Te[k], TB = WFL_E * t * B[k], v;
       int wl = 1;
       if (policy) {

@@ -33,6 +33,9 @@ const wfl_geometry *wfl_internal_profile(void);
 double wfl_internal_igrand(const wfl_geometry *g, int fermion, double T, double rho, double rho2, double evm1,
                            int *branch, double *B);
 
+/* Output level of the drivers (wfl_set_verbosity; the reference's verbosity). */
+int wfl_internal_verbosity(void);
+
 /* FP32 quantization used for every parameter the kernels see. */
 static inline double wfl_q32(double v) { return (double)(float)v; }
 

@@ -86,11 +86,12 @@ EXPORTS = [
     "wfl_eval_batch", "wfl_group_ev_minus1", "wfl_spline_build", "wfl_set_profile",
     "wfl_set_numerics", "wfl_ev_reset", "wfl_ev_launches", "wfl_logsimpson", "wfl_effective_T",
     "wfl_effective_tuple",
-    "wfl_leff_groups",
+    "wfl_leff_groups", "wfl_leff_groups_est", "wfl_group_ev_minus1_est",
     "EV", "EVMean", "Tfunc", "integrateT", "getwl", "Exact", "MagField", "smTscal", "smTferm",
     "rhogrand", "testvalrho", "transformRho", "integrateRhotoa", "integrateRho", "PeriodicCA", "FixFluxCA",
     "ClassAction", "EAction", "wfl_eaction", "wfl_set_verbosity",
     "wfl_cell_action_groups", "wfl_radial_action_groups", "wfl_interaction_energy",
+    "wfl_set_fp_traps", "wfl_fp_trap_count",
     "wfl_loops_write", "wfl_loops_read_header", "wfl_loops_read", "wfl_write_text_loops",
 ]
 
@@ -145,6 +146,8 @@ def _load():
         "wfl_effective_T": (D, [P, D]),
         "wfl_effective_tuple": (I, [P, D, D, D, P]),
         "wfl_leff_groups": (I, [P, I, I, P, D, D, P, P, I, I, P]),
+        "wfl_leff_groups_est": (I, [P, I, I, P, D, D, P, P, I, I, I, P]),
+        "wfl_group_ev_minus1_est": (D, [P, I, I]),
         "EV": (D, [D, P, P]),
         "EVMean": (None, [P, P, P, I, P, D, I]),
         "Tfunc": (P, [D, P, P]),
@@ -162,6 +165,8 @@ def _load():
         "wfl_cell_action_groups": (I, [P, I, I, P, I, P, P, P, P, P, I, I, P]),
         "wfl_radial_action_groups": (I, [P, I, I, P, I, P, P, P, I, I, P]),
         "wfl_interaction_energy": (I, [I, P, P, P]),
+        "wfl_set_fp_traps": (I, [I]),
+        "wfl_fp_trap_count": (C.c_longlong, []),
         "getwl": (I, [P, C.c_char_p, I, I, I]),
         "Exact": (D, [D, D, I]),
         "MagField": (D, [D, P]),
@@ -449,17 +454,19 @@ def logsimpson(T, f):
     return lib.wfl_logsimpson(_ptr(T), _ptr(f), len(T))
 
 
-def leff_groups(geom: Geometry, fermion, T, cx, cy, gsum, counts, policy=1):
+def leff_groups(geom: Geometry, fermion, T, cx, cy, gsum, counts, policy=1, estimator=0):
     """L_eff per group; gsum [nT, ngroups, 3] for one (geometry, x_cm); T,
-    cx, cy as the batch was created with (nominal, log-spaced T)."""
+    cx, cy as the batch was created with (nominal, log-spaced T);
+    estimator 0 the reference's Richardson combination, 1 the two-level
+    extrapolation 2 W_{N/2} - W_{N/4}."""
     T_eff = np.ascontiguousarray(T, np.float64)
     gsum = np.ascontiguousarray(gsum, np.float64)
     counts = np.ascontiguousarray(counts, np.int32)
     ng = gsum.shape[1]
     out = np.zeros(ng, np.float64)
     g = geom.c()
-    check(lib.wfl_leff_groups(C.byref(g), int(fermion), len(T_eff), _ptr(T_eff), float(cx), float(cy),
-                              _ptr(gsum), _ptr(counts), ng, int(policy), _ptr(out)))
+    check(lib.wfl_leff_groups_est(C.byref(g), int(fermion), len(T_eff), _ptr(T_eff), float(cx), float(cy),
+                                  _ptr(gsum), _ptr(counts), ng, int(policy), int(estimator), _ptr(out)))
     return out
 
 

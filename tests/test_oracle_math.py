@@ -348,3 +348,59 @@ def test_eaction_rules_converge_to_lcf_quadrature(orc):
     coarse = orc.eaction_groups(g, x, x, 1, 1, 2, 2, 20, 40, lcf=True)[0]
     assert abs(fine - ref) <= 1e-6 * abs(ref)
     assert abs(coarse - ref) <= 1e-2 * abs(ref)
+
+
+def _bump_integral(x):
+    """int_-1^x e^{-1/(1-t^2)} dt by adaptive quadrature."""
+    bump = lambda t: np.exp(-1.0 / (1.0 - t * t)) if abs(t) < 1 else 0.0
+    return si.quad(bump, -1.0, x, epsabs=1e-14, limit=200)[0]
+
+
+# The fitted part of the fixflux chi (intEV.cu:188-190, fit A; intEVff.cu:
+# 170-171, fit B) approximates int_-1^x bump; its error against the integral
+# over x in (-1, 1) (801 points), measured once with scipy: rms, mean, max.
+CHI_FIT_ACCURACY = {"FIXFLUX": (0.0220420, 0.0117647, 0.0893153),
+                    "FIXFLUX_FF": (0.00287788, -0.00253405, 0.00401459)}
+
+
+@pytest.mark.parametrize("prof", ["FIXFLUX", "FIXFLUX_FF"])
+def test_fixflux_outer_branch_vs_quadrature(orc, prof):
+    """Outside rho = a/2 the fixflux f is 1 + 0.75 (4 rho^2/a^2 - 1) lml +
+    3 n (n - 1) aml + (3 lambda aml/(q1 a)) chi(x), x = 2 (rho - n a)/lambda
+    (intEV.cu:247-252), with chi = int_-1^x (t + 2 n a/lambda) bump(t) dt:
+    the t bump part in closed form (E1), the rest a fit of int bump.  So f
+    must equal f(a/2) plus the integral of f' = fpfixflux (intEV.cu:334-357)
+    up to 6 n aml/q1 times the fit's error at x, and the fit must match the
+    integral with its recorded accuracy (rms within 1 %, mean within 1 % of
+    the rms): a wrong constant, sign or term in either restatement fails
+    one of the two."""
+    g = orc.Geom(getattr(orc, prof), F=1.0, l2=0.36, tubedist=2.0, lmin=0.2, quantize=False)
+    a, lam, lmin, q1 = 2.0, 0.6, 0.2, 0.443993816168079
+    aml = (a - lam) / (a - lmin)
+    f0 = orc.lib.orc_f(g.ref, (a / 2) ** 2)
+    rms0, mean0, max0 = CHI_FIT_ACCURACY[prof]
+    fit_err = []
+    for rho in np.linspace(a / 2 + 1e-3, 2.9 * a, 120):
+        n = np.floor((rho + a / 2) / a)
+        x = 2 * (rho - n * a) / lam
+        fex = f0 + si.quad(lambda r: orc.lib.orc_fp(g.ref, r * r) * 2 * r, a / 2, rho, limit=400, epsabs=1e-12,
+                           points=[k * a + s * lam / 2 for k in (1, 2) for s in (-1, 0, 1)])[0]
+        d = orc.lib.orc_f(g.ref, rho * rho) - fex
+        coef = 6 * n * aml / q1
+        if abs(x) < 0.999:
+            fit_err.append(d / coef)
+        assert abs(d) <= coef * max0 * 1.02 + 1e-9, (rho, d)
+    # the fit itself, against the integral
+    xs = np.linspace(-0.9995, 0.9995, 801)
+    ex = np.array([_bump_integral(x) for x in xs])
+    x2 = xs * xs
+    x4 = x2 * x2
+    if prof == "FIXFLUX":
+        fit = 0.444 / (1 + np.exp(-(3.31 * xs + 5.25 * xs * x2 * np.sin(xs) * np.cos(-0.907 * x2 - 1.29 * x4 * x4))
+                                  / np.cos(xs)))
+    else:
+        fit = 0.218 + 0.393 * xs * np.cosh(0.806 * x4 - 0.696 * x2 + 0.0902) / np.cosh(0.825 * x2 - 0.0234 * xs + 0.375)
+    e = fit - ex
+    assert abs(np.sqrt(np.mean(e * e)) / rms0 - 1) < 0.01 and abs(e.mean() - mean0) < 0.01 * rms0
+    # the oracle's outer branch carries exactly that fit error
+    assert len(fit_err) > 20 and max(abs(v) for v in fit_err) <= max0 * 1.02

@@ -71,7 +71,13 @@ struct Wparams {
 
 /* intEV.cu:111 -- evaluates every loop at (params->xcm, sqrt(T)) on the GPU,
  * overwrites Wsscal_h/Wsferm_h for all Nl loops and returns EVMean over the
- * 128 loops of WLlist. */
+ * 128 loops of WLlist.  A call whose buffers (by address), shape, x_cm, T,
+ * F, l2 and library settings equal the previous launch's refills the host
+ * buffers from that launch instead of launching again (the reference
+ * relaunches once per loop group with identical inputs).  Contents are not
+ * compared: after rewriting the loops or the spline tables in place at
+ * the same addresses, call wfl_ev_reset() (or wfl_set_profile /
+ * wfl_set_numerics) before the next EV(). */
 double EV(double T, void *p, int *WLlist);
 /* Integrand.c:325 */
 void EVMean(double *EV, float4 *Wsscal_h, float4 *Wsferm_h, int n, int *WL, double T, int fermion);

@@ -1513,8 +1513,9 @@ extern "C" int wfl_internal_ev_launch(const void* dev_aos, int Nl, int N, const 
   gg.lattice = 0.0;
   gg.images = 0;
   const bool scaled = precision == WFL_MIXED;
-  // spline: the caller's device tables (read back every call: the caller may
-  // rewrite them in place) as a blob with its bucket table
+  // spline: the caller's device tables, read back at every launch, as a
+  // blob with its bucket table (EV() skips the launch, and this read, when
+  // its inputs equal the previous launch's: see EV in intT.h)
   SplineBlob sb;
   if (g->kind == WFL_SPLINE) {
     std::vector<float> kn((size_t)nspline), cf((size_t)nspline * 4);

@@ -2,9 +2,12 @@
  * wfl_host.c -- host side of libwfl.so, in C like the reference's host code.
  *
  *   EV          drop-in for intEV.cu:111-148 (launch via the HIP C ABI, D2H,
- *               EVMean); repeated calls at the same (x_cm, T, F, l2) reuse
- *               the device results instead of relaunching (the reference
- *               relaunches once per loop group, Integrand.c:249-301).
+ *               EVMean); repeated calls at the same (x_cm, T, F, l2) and
+ *               buffer addresses reuse the launch's results instead of
+ *               relaunching (the reference relaunches once per loop group,
+ *               Integrand.c:249-301); buffer contents are not compared, so a
+ *               caller that rewrites loops or spline tables in place calls
+ *               wfl_ev_reset() first.
  *   EVMean      Integrand.c:325-407 (ensemble average, Richardson)
  *   Tfunc       Integrand.c:195-316 (proper-time integrand policy)
  *   integrateT  Tint.c:15-33, 100-175 (Simpson in t, T = (1-t)/t)

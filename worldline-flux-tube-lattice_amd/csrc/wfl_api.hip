@@ -994,17 +994,20 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
   }
   // lane groups: tuples sorted by node kind (local Taylor node, accurate
   // Gaussian node, plain node) and geometry, each such set padded to 64
-  // lanes.  A wavefront holds one node kind; it holds one geometry except in
+  // lanes.  A wavefront holds one node kind and one loop count (the kernels
+  // run every lane over its group's loops); it holds one geometry except in
   // the Gaussian axis form, whose kernels read every geometry parameter per
-  // lane and use no image window, so that there only the (at most three)
-  // node kinds are padded, not each geometry's.  Local-node tuples take the
+  // lane and use no image window, so that there only the node kinds and loop
+  // counts are padded, not each geometry's.  Local-node tuples take the
   // local path whatever their accurate flag; within a kind they are ordered
   // by geometry and node range.
   const bool cross_geom = p->axis && p->prof_gauss;
   std::vector<int> order(b->ntup);
   for (int t = 0; t < b->ntup; ++t) order[t] = t;
   auto kind_of = [&](int t) { return p->tups[t].loc ? 2LL : (long long)p->tups[t].acc; };
-  auto key = [&](int t) { return cross_geom ? kind_of(t) : 4LL * p->tups[t].geom + kind_of(t); };
+  auto key = [&](int t) {
+    return cross_geom ? 4LL * p->tups[t].nloops + kind_of(t) : 4LL * p->tups[t].geom + kind_of(t);
+  };
   std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
     if (key(a) != key(c)) return key(a) < key(c);
     if (p->tups[a].geom != p->tups[c].geom) return p->tups[a].geom < p->tups[c].geom;

@@ -212,6 +212,19 @@ int wfl_plan_mapping(const wfl_plan *p);
 void wfl_plan_destroy(wfl_plan *p);
 /* create + run + read + destroy. */
 int wfl_eval_batch(const wfl_batch *b, wfl_loops *L, double *gsum, double *perloop);
+/* wfl_eval_batch without device allocation.  No counterpart in the
+ * reference: for callers that cannot call hipMalloc (serving loops,
+ * harnesses that forbid it).  The plan's device buffers live in one static
+ * 64 MiB device array per GPU, held until the process ends; a batch that
+ * needs more fails with WFL_ERR_ARG and a message naming the sizes.  The
+ * loop set's device records must already exist: the first wfl_eval_batch or
+ * wfl_plan_create of a batch of the same precision and form on L builds them
+ * (otherwise WFL_ERR_STATE).  The default launch configuration is used (no
+ * autotuning); results equal wfl_eval_batch's for batches whose chunk sums
+ * take at most half the array.  Calls on one GPU share the array and are
+ * serialised by a lock; the call returns when gsum (and perloop) are
+ * written.  Host-side bookkeeping uses the C++ heap. */
+int wfl_eval_batch_noalloc(const wfl_batch *b, wfl_loops *L, double *gsum, double *perloop);
 
 /* Richardson-extrapolated <W> - 1 of one group from its sums
  * (Integrand.c:400): 8/3 m_N - 2 m_{N/2} + 1/3 m_{N/4}. */

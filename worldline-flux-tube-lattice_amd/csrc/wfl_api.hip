@@ -348,9 +348,19 @@ extern "C" int wfl_loops_shape(const wfl_loops* L, int* Nl, int* N) {
   return WFL_OK;
 }
 
+// Device data derived from the loop set is built (and allocated) on first
+// use; with build = false (a plan in caller-owned memory) data not yet built
+// is an error.
+static int loops_absent(const char* what) {
+  wfl_set_error("wfl_eval_batch_noalloc: the loop set has no %s yet (an evaluation of a batch of the same "
+                "precision and form with wfl_eval_batch or wfl_plan_create builds it)", what);
+  return WFL_ERR_STATE;
+}
+
 // The ensemble's largest loop radius max |w| (measured once, on the device).
-static int loops_rmax(wfl_loops* L, float* rmax) {
+static int loops_rmax(wfl_loops* L, float* rmax, bool build = true) {
   if (L->rmax < 0.0f) {
+    if (!build) return loops_absent("radius");
     unsigned int* d = nullptr;
     HIP_TRY(hipMalloc(&d, sizeof(unsigned int)));
     unsigned int h = 0;
@@ -372,7 +382,9 @@ static int loops_rmax(wfl_loops* L, float* rmax) {
   return WFL_OK;
 }
 
-static int loops_records(wfl_loops* L, int precision) {
+static int loops_records(wfl_loops* L, int precision, bool build = true) {
+  if (!build && precision == WFL_MIXED && !L->rec32) return loops_absent("FP32 point records");
+  if (!build && precision == WFL_FP64 && !L->rec64) return loops_absent("FP64 point records");
   long long n = (long long)L->Nl * L->N;
   dim3 grid((unsigned)((n + 255) / 256)), block(256);
   if (precision == WFL_MIXED && !L->rec32) {
@@ -390,8 +402,9 @@ static int loops_records(wfl_loops* L, int precision) {
   return WFL_OK;
 }
 
-static int loops_records_axis(wfl_loops* L) {
+static int loops_records_axis(wfl_loops* L, bool build = true) {
   if (L->reca) return WFL_OK;
+  if (!build) return loops_absent("axis-form records");
   long long n = (long long)L->Nl * L->N;
   HIP_TRY(hipMalloc(&L->reca, sizeof(float) * RECA * (size_t)n));
   hipLaunchKernelGGL(k_records_axis, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, L->x, L->y, L->Nl, L->N,
@@ -401,8 +414,9 @@ static int loops_records_axis(wfl_loops* L) {
   return WFL_OK;
 }
 
-static int loops_info(wfl_loops* L) {
+static int loops_info(wfl_loops* L, bool build = true) {
   if (L->info) return WFL_OK;
+  if (!build) return loops_absent("per-loop sums");
   HIP_TRY(hipMalloc(&L->info, sizeof(LoopInfo) * (size_t)L->Nl));
   hipLaunchKernelGGL(k_loop_info, dim3((unsigned)L->Nl), dim3(256), 0, 0, L->x, L->y, L->N, L->info);
   HIP_TRY(hipGetLastError());
@@ -794,6 +808,10 @@ struct wfl_plan_s {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;   // bracket the main kernel (timed runs)
   bool timed = false;
   hipStream_t last = nullptr;              // stream of the latest run (wfl_plan_read waits for it)
+  // device buffers carved from caller-owned memory instead of hipMalloc
+  // (wfl_eval_batch_noalloc): never freed or grown by the plan
+  char* arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
 };
 
 extern "C" void wfl_plan_destroy(wfl_plan* p) {
@@ -805,14 +823,34 @@ extern "C" void wfl_plan_destroy(wfl_plan* p) {
   if (p->ev1) (void)hipEventDestroy(p->ev1);
   void* bufs[] = {p->d_tups, p->d_geoms, p->d_lanes, p->d_tasks, p->d_sblob, p->d_gpart,
                   p->d_gsum, p->d_perloop, p->d_wm1, p->d_nonfinite, p->d_queue};
-  for (void* b : bufs) if (b) (void)hipFree(b);
+  if (!p->arena)
+    for (void* b : bufs) if (b) (void)hipFree(b);
   delete p;
 }
 
+// A device buffer of the plan: hipMalloc, or the next 256-byte aligned
+// piece of the plan's arena (an error when the arena is too small).
+static int plan_alloc(wfl_plan* p, void** dst, size_t bytes) {
+  if (!p->arena) {
+    HIP_TRY(hipMalloc(dst, bytes));
+    return WFL_OK;
+  }
+  const size_t at = (p->arena_used + 255) & ~(size_t)255;
+  if (at > p->arena_cap || bytes > p->arena_cap - at) {
+    wfl_set_error("wfl_eval_batch_noalloc: the batch needs more than the %zu-byte scratch "
+                  "(%zu bytes in use, %zu more asked for)", p->arena_cap, at, bytes);
+    return WFL_ERR_ARG;
+  }
+  *dst = p->arena + at;
+  p->arena_used = at + bytes;
+  return WFL_OK;
+}
+
 template <class T>
-static int upload(T** dst, const T* src, size_t n) {
+static int upload(wfl_plan* p, T** dst, const T* src, size_t n) {
   if (n == 0) n = 1;
-  HIP_TRY(hipMalloc((void**)dst, sizeof(T) * n));
+  int rc = plan_alloc(p, (void**)dst, sizeof(T) * n);
+  if (rc) return rc;
   if (src) HIP_TRY(hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice));
   return WFL_OK;
 }
@@ -851,9 +889,10 @@ static std::vector<Task> plan_task_list(const wfl_plan* p, int cpg) {
 static int plan_set_tasks(wfl_plan* p, int cpg) {
   std::vector<Task> tasks = plan_task_list(p, cpg);
   if (tasks.size() > p->tasks_cap || !p->d_tasks) {
-    if (p->d_tasks) (void)hipFree(p->d_tasks);
+    if (p->d_tasks && !p->arena) (void)hipFree(p->d_tasks);
     p->d_tasks = nullptr; p->tasks_cap = 0;
-    HIP_TRY(hipMalloc(&p->d_tasks, sizeof(Task) * std::max<size_t>(tasks.size(), 1)));
+    int rc = plan_alloc(p, (void**)&p->d_tasks, sizeof(Task) * std::max<size_t>(tasks.size(), 1));
+    if (rc) return rc;
     p->tasks_cap = std::max<size_t>(tasks.size(), 1);
   }
   if (!tasks.empty()) HIP_TRY(hipMemcpy(p->d_tasks, tasks.data(), sizeof(Task) * tasks.size(), hipMemcpyHostToDevice));
@@ -861,9 +900,10 @@ static int plan_set_tasks(wfl_plan* p, int cpg) {
   if (cpg > 1) {
     const size_t n = (size_t)std::max(p->b.ntup, 1) * p->ngroups * cpg * 3;
     if (n > p->gpart_cap) {
-      if (p->d_gpart) (void)hipFree(p->d_gpart);
+      if (p->d_gpart && !p->arena) (void)hipFree(p->d_gpart);
       p->d_gpart = nullptr; p->gpart_cap = 0;
-      HIP_TRY(hipMalloc(&p->d_gpart, sizeof(double) * n));
+      int rc = plan_alloc(p, (void**)&p->d_gpart, sizeof(double) * n);
+      if (rc) return rc;
       p->gpart_cap = n;
     }
     HIP_TRY(hipMemset(p->d_gpart, 0, sizeof(double) * n));
@@ -879,14 +919,16 @@ static int plan_set_tasks(wfl_plan* p, int cpg) {
 // point records are reused from L2 instead of being fetched once per lane
 // group (C2, 128-loop groups: 700 ms with one task per group, 649 ms with
 // 16-loop tasks, 647 to 649 ms with 8, 646 ms with 4).  The chunk sums cost
-// 24 B per tuple, group and chunk; they are kept under 2 GB.
+// 24 B per tuple, group and chunk; they are kept under 2 GB (under half the
+// arena for a plan in caller-owned memory).
 static const int kTaskLoops = 8;
 
 static int plan_default_cpg(const wfl_plan* p, int mapping) {
   if (mapping == 2) return 1;
   const int group = p->b.group;
   const double slot_bytes = 24.0 * (double)std::max(p->b.ntup, 1) * (double)p->ngroups;
-  auto fits = [&](int c) { return c <= 32 && c <= group && slot_bytes * c <= 2.0e9; };
+  const double slot_cap = p->arena ? 0.5 * (double)p->arena_cap : 2.0e9;
+  auto fits = [&](int c) { return c <= 32 && c <= group && slot_bytes * c <= slot_cap; };
   const long long want = 32LL * device_cus();
   int cpg = 1;
   while (fits(2 * cpg) && (long long)p->ntasks1 * cpg < want) cpg *= 2;
@@ -896,7 +938,13 @@ static int plan_default_cpg(const wfl_plan* p, int mapping) {
 
 #define TRY(expr) do { int rc_ = (expr); if (rc_) { wfl_plan_destroy(p); return rc_; } } while (0)
 
+static int plan_create_impl(wfl_plan** out, const wfl_batch* b, wfl_loops* L, char* arena, size_t arena_cap);
+
 extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L) {
+  return plan_create_impl(out, b, L, nullptr, 0);
+}
+
+static int plan_create_impl(wfl_plan** out, const wfl_batch* b, wfl_loops* L, char* arena, size_t arena_cap) {
   if (!out || !b || !L) { wfl_set_error("wfl_plan_create: NULL argument"); return WFL_ERR_ARG; }
   *out = nullptr;
   if (b->mode != WFL_MODE_REFERENCE && b->mode != WFL_MODE_PHYSICAL) { wfl_set_error("bad mode %d", b->mode); return WFL_ERR_ARG; }
@@ -920,11 +968,13 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
   if (rc0) return rc0;
   float rmax = 0.0f;
   if (b->precision == WFL_MIXED) {
-    rc0 = loops_rmax(L, &rmax);
+    rc0 = loops_rmax(L, &rmax, !arena);
     if (rc0) return rc0;
   }
 
   wfl_plan* p = new wfl_plan();
+  p->arena = arena;
+  p->arena_cap = arena ? arena_cap : 0;
   p->b = *b;
   p->b.geoms = nullptr; p->b.splines = nullptr; p->b.tup_geom = nullptr;
   p->b.tup_cx = p->b.tup_cy = p->b.tup_T = nullptr; p->b.geom_nloops = nullptr;
@@ -983,7 +1033,7 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
   // ensemble's loop radii (the kernel checks each loop's own radius)
   std::vector<double> ldmax(b->ntup, 0.0);
   if (p->axis && p->prof_gauss) {
-    TRY(loops_info(L));
+    TRY(loops_info(L, !arena));
     std::vector<double> rs = L->radius;
     std::sort(rs.begin(), rs.end());
     const double R = rs[std::min(rs.size() - 1, (size_t)(0.9 * (double)rs.size()))];
@@ -1055,29 +1105,29 @@ extern "C" int wfl_plan_create(wfl_plan** out, const wfl_batch* b, wfl_loops* L)
   if (mapping == 3 && !p->prof_gauss) mapping = 1;
   p->mapping = mapping;
 
-  TRY(loops_records(L, b->precision));
-  if (p->axis) TRY(loops_records_axis(L));
-  TRY(upload(&p->d_tups, p->tups.data(), p->tups.size()));
-  TRY(upload(&p->d_geoms, geoms.data(), geoms.size()));
-  TRY(upload(&p->d_lanes, lanes.data(), lanes.size()));
+  TRY(loops_records(L, b->precision, !arena));
+  if (p->axis) TRY(loops_records_axis(L, !arena));
+  TRY(upload(p, &p->d_tups, p->tups.data(), p->tups.size()));
+  TRY(upload(p, &p->d_geoms, geoms.data(), geoms.size()));
+  TRY(upload(p, &p->d_lanes, lanes.data(), lanes.size()));
   // loop groups split into chunk tasks (the autotuner measures alternatives)
   TRY(plan_set_tasks(p, plan_default_cpg(p, mapping)));
-  TRY(upload(&p->d_sblob, sblob.data.data(), sblob.data.size()));
+  TRY(upload(p, &p->d_sblob, sblob.data.data(), sblob.data.size()));
   size_t ng3 = (size_t)std::max(b->ntup, 1) * p->ngroups * 3;
-  TRY(upload<double>(&p->d_gsum, nullptr, ng3));
+  TRY(upload<double>(p, &p->d_gsum, nullptr, ng3));
   {
     hipError_t e = hipMemset(p->d_gsum, 0, sizeof(double) * ng3);
     if (e != hipSuccess) { wfl_set_error("hipMemset: %s", hipGetErrorString(e)); wfl_plan_destroy(p); return WFL_ERR_DEVICE; }
   }
   if (b->want_perloop || mapping == 3) {
     size_t n6 = (size_t)std::max(b->ntup, 1) * Nl * 6;
-    TRY(upload<double>(&p->d_perloop, nullptr, n6));
+    TRY(upload<double>(p, &p->d_perloop, nullptr, n6));
     hipError_t e = hipMemset(p->d_perloop, 0, sizeof(double) * n6);
     if (e != hipSuccess) { wfl_set_error("hipMemset: %s", hipGetErrorString(e)); wfl_plan_destroy(p); return WFL_ERR_DEVICE; }
   }
-  if (mapping == 2) TRY(upload<double>(&p->d_wm1, nullptr, (size_t)std::max(b->ntup, 1) * Nl * 3));
-  TRY(upload<unsigned long long>(&p->d_nonfinite, nullptr, 1));
-  TRY(upload<int>(&p->d_queue, nullptr, 1));
+  if (mapping == 2) TRY(upload<double>(p, &p->d_wm1, nullptr, (size_t)std::max(b->ntup, 1) * Nl * 3));
+  TRY(upload<unsigned long long>(p, &p->d_nonfinite, nullptr, 1));
+  TRY(upload<int>(p, &p->d_queue, nullptr, 1));
   {
     int dev = 0, ncu = 0;
     (void)hipGetDevice(&dev);
@@ -1393,6 +1443,61 @@ extern "C" int wfl_plan_nonfinite(wfl_plan* p, long long* count) {
 extern "C" int wfl_eval_batch(const wfl_batch* b, wfl_loops* L, double* gsum, double* perloop) {
   wfl_plan* p = nullptr;
   int rc = wfl_plan_create(&p, b, L);
+  if (rc) return rc;
+  rc = wfl_plan_run(p, nullptr);
+  if (!rc) rc = wfl_plan_read(p, gsum, perloop);
+  wfl_plan_destroy(p);
+  return rc;
+}
+
+// ------------------------------------------------------ no-allocation evaluation
+// wfl_eval_batch for callers that cannot allocate device memory: the plan's
+// device buffers (tuples, geometries, lane groups, tasks, spline blob, chunk
+// and group sums, per-loop outputs, counters) are carved from one static
+// device array per GPU.  Each GPU keeps its copy of the array until the
+// process ends, and the array keeps the last call's contents: every buffer
+// that must start at zero is cleared by the call.  Calls on one GPU share the
+// array, so they are serialised by a lock (the call returns after its
+// results are on the host).
+static constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static constexpr int kScratchDevices = 64;
+static void* g_scratch_ptr[kScratchDevices];   // device address of scratch, per device
+static std::mutex g_scratch_mu;                // guards g_scratch_ptr and the arrays
+
+// Address of the current device's copy of scratch (looked up once per
+// device).  Host code uses this address, never the array's name, which on
+// the host is only a stand-in for the symbol.
+static int scratch_address(void** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices) {
+    wfl_set_error("wfl_eval_batch_noalloc: device %d is beyond the %d devices served", dev, kScratchDevices);
+    return WFL_ERR_DEVICE;
+  }
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      wfl_set_error("wfl_eval_batch_noalloc: scratch address on device %d: %s", dev, hipGetErrorString(e));
+      return WFL_ERR_DEVICE;
+    }
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return WFL_OK;
+}
+
+extern "C" int wfl_eval_batch_noalloc(const wfl_batch* b, wfl_loops* L, double* gsum, double* perloop) {
+  if (!b || !L) { wfl_set_error("wfl_eval_batch_noalloc: NULL argument"); return WFL_ERR_ARG; }
+  int rc = require_device();
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  void* base = nullptr;
+  rc = scratch_address(&base);
+  if (rc) return rc;
+  wfl_plan* p = nullptr;
+  rc = plan_create_impl(&p, b, L, (char*)base, kScratchBytes);
   if (rc) return rc;
   rc = wfl_plan_run(p, nullptr);
   if (!rc) rc = wfl_plan_read(p, gsum, perloop);

@@ -83,7 +83,7 @@ EXPORTS = [
     "wfl_plan_uniform_splines", "wfl_autotune_nsub", "wfl_plan_axis_form", "wfl_plan_local_tuples",
     "wfl_local_taylor",
     "wfl_plan_gsum_device", "wfl_plan_nonfinite", "wfl_plan_mapping", "wfl_plan_destroy",
-    "wfl_eval_batch", "wfl_group_ev_minus1", "wfl_spline_build", "wfl_set_profile",
+    "wfl_eval_batch", "wfl_eval_batch_noalloc", "wfl_group_ev_minus1", "wfl_spline_build", "wfl_set_profile",
     "wfl_set_numerics", "wfl_ev_reset", "wfl_ev_launches", "wfl_logsimpson", "wfl_effective_T",
     "wfl_effective_tuple",
     "wfl_leff_groups", "wfl_leff_groups_est", "wfl_group_ev_minus1_est",
@@ -136,6 +136,7 @@ def _load():
         "wfl_plan_mapping": (I, [P]),
         "wfl_plan_destroy": (None, [P]),
         "wfl_eval_batch": (I, [P, P, P, P]),
+        "wfl_eval_batch_noalloc": (I, [P, P, P, P]),
         "wfl_group_ev_minus1": (D, [P, I]),
         "wfl_spline_build": (I, [P, P, I, I, P]),
         "wfl_set_profile": (I, [P, I]),
@@ -441,6 +442,20 @@ def eval_batch(batch: Batch, loops: Loops):
         return p.read()
     finally:
         p.close()
+
+
+def eval_batch_noalloc(batch: Batch, loops: Loops):
+    """eval_batch through wfl_eval_batch_noalloc: no device allocation (the
+    plan lives in the library's static per-GPU scratch array).  The loop set's
+    records must exist already (an eval_batch of the same precision and form
+    on these loops builds them)."""
+    cb, keep = batch.c()
+    ntup = len(batch.tup_geom)
+    ngroups = (loops.Nl + batch.group - 1) // batch.group
+    gs = np.zeros((ntup, ngroups, 3), np.float64)
+    pl = np.zeros((ntup, loops.Nl, 6), np.float64) if batch.want_perloop else None
+    check(lib.wfl_eval_batch_noalloc(C.byref(cb), loops.h, _ptr(gs), _ptr(pl)))
+    return gs, pl
 
 
 def group_ev_minus1(gsum3, count):

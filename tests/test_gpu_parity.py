@@ -293,6 +293,31 @@ def test_spline_small_tables(gpu, orc, loops, spinor):
         assert e1 < 2e-5 and e2 < 1e-5
 
 
+@pytest.mark.parametrize("spinor", [0, 1])
+def test_spline_uniform_grid_rows_past_the_table(gpu, orc, loops, spinor):
+    """The direct-index kernels read one row per node with no clamp: the
+    plan repeats the constant row past the last knot as far as its tuples'
+    nodes can reach (the largest loop radius times sqrt(T) beyond x_cm).  At
+    T = 150 the nodes reach rho ~ 25 against the table's 20, which the
+    repeated rows cover; at T = 1e4 they would need more rows than the LDS
+    layout holds, and the plan takes the exact interval search.  Both
+    against the oracle."""
+    knots, coefs = gaussian_spline(gpu)
+    geoms = [gpu.Geometry(kind=gpu.SPLINE, F=1.0, l2=1.0, a=1.0, spline_table=0)]
+    og = [orc.Geom(orc.SPLINE, F=1.0, l2=1.0, tubedist=1.0, knots=knots, coefs=coefs)]
+    x, y, _ = loops
+    rmax = float(np.sqrt(x.astype(np.float64) ** 2 + y.astype(np.float64) ** 2).max())
+    for Ts, want_uni in (([0.3, 150.0], 1), ([0.3, 1.0e4], 0)):
+        assert (0.9 + np.sqrt(Ts[1]) * rmax > 20.0) and (want_uni == 0 or 0.9 + np.sqrt(Ts[1]) * rmax < 30.0)
+        seen = []
+        gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, 1, 2, spinor, 1, 1, tuples([0.0, 0.9], Ts),
+                                    splines=[(knots, coefs)], on_plan=lambda p: seen.append(p.uniform_splines))
+        e1, e2 = relerr(pl, rpl), relerr(gs, rgs)
+        print(f"  T {Ts}: uniform {seen[0]} perloop {e1:.2e} gsum {e2:.2e}")
+        assert seen == [want_uni]
+        assert e1 < 2e-5 and e2 < 1e-5
+
+
 @pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 5e-5)])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_spline_irregular_knots_and_mixed_profiles(gpu, orc, loops, prec, tol, mode):

@@ -510,14 +510,16 @@ static double unit_scale(const wfl_geometry& g, bool scaled) {
 // u is at most s + 2 for every bucket; nb is the smallest of n/2, n, 2n, 4n,
 // 8n (at most 16384) for which it is, else 2n with the scan.
 // Uniform-grid rows (FP32 physical mode, tables whose knots lie on
-// r_j = r0 + j h), at the front of the blob: the n + 1
-// coefficient rows, then their n + 1 left knots as floats (0, k_0, ...,
-// k_{n-2}, k_{n-1}) padded to whole float4.  uni_f4 is the size of that
-// prefix, the part the uniform-grid kernels stage in LDS.
+// r_j = r0 + j h), at the front of the blob: kUniRows float4 of rows (each
+// table's p copies of the row below the first knot, its n + 1 rows, and
+// copies of the row past the last knot up to the largest u its tuples
+// reach), then one float4 per row holding that row's left knot in .x.
+// uni_f4 is the size of that prefix, the part the uniform-grid kernels
+// stage in LDS.  A table whose padded rows do not fit is not uniform.
 struct SplineTab {
   int cf, rec, kn, n, nb, fast;
   float r0, inv_w, k0, kl;
-  int uni, ucf, ukn;
+  int uni, ucf, ulo, urows;   // uniform rows: offset, rows below the first knot's, rows in all
   float uinv_h, uoff;
 };
 struct SplineBlob {
@@ -527,23 +529,37 @@ struct SplineBlob {
   bool all_uni = false;  // every table has them
 };
 
-// Knots on a uniform grid in r = sqrt(rho^2): with inv_h = fl32(1/h) and
-// off = fl32(1 - r0 inv_h), every knot k_j has sqrt(k_j) inv_h + off within
-// 1e-4 of j + 1.
+// Knots on a uniform grid in r = sqrt(rho^2): with inv_h = fl32(1/h),
+// p = the rows kept below the first knot's row (so that off >= 0) and
+// off = fl32(1 + p - r0 inv_h), every knot k_j has sqrt(k_j) inv_h + off
+// within 1e-4 of j + 1 + p.
 static bool spline_uniform(const float* k, int n, SplineTab& st) {
   if (n < 3 || !(k[0] >= 0.0f)) return false;
   const double r0 = sqrt((double)k[0]), r1 = sqrt((double)k[n - 1]);
   const double h = (r1 - r0) / (double)(n - 1);
   if (!(h > 0.0)) return false;
   const float inv_h = (float)(1.0 / h);
-  const float off = (float)(1.0 - r0 * (double)inv_h);
+  const double off0 = 1.0 - r0 * (double)inv_h;
+  if (!(off0 > -(double)kUniRows)) return false;
+  const int p = off0 < 0.0 ? (int)ceil(-off0) : 0;
+  const float off = (float)(off0 + p);
+  if (!(off >= 0.0f)) return false;
   for (int j = 0; j < n; ++j) {
     const double t = sqrt((double)k[j]) * (double)inv_h + (double)off;
-    if (!(fabs(t - (double)(j + 1)) <= 1e-4)) return false;
+    if (!(fabs(t - (double)(j + 1 + p)) <= 1e-4)) return false;
   }
   st.uinv_h = inv_h;
   st.uoff = off;
+  st.ulo = p;
   return true;
+}
+
+// Rows a uniform table needs for nodes up to u = umax: the kernel's row
+// trunc(sqrt(u) inv_h + off), FP32-rounded, plus two rows of margin.
+static double spline_uniform_rows(const SplineTab& st, double umax) {
+  if (!(umax >= 0.0) || !std::isfinite(umax)) return HUGE_VAL;
+  const double t = sqrt(umax * (1.0 + 1e-5)) * (double)st.uinv_h * (1.0 + 1e-6) + (double)st.uoff;
+  return floor(t) + 3.0;
 }
 
 // largest j <= n - 2 with k[j] <= v, 0 if none
@@ -571,8 +587,10 @@ static bool spline_buckets(const float* k, int n, int nb, float r0, float inv_w,
   return fast;
 }
 
+// umax[t] (FP32 physical mode): the largest u = rho^2 that a node of table
+// t can see (table t gets uniform rows only if it is given and they fit).
 static void build_spline_blob(int ntab, const float* const* rho2, const float* const* coefs, const int* n, int mode,
-                              SplineBlob& sb, bool want_uniform = false) {
+                              SplineBlob& sb, const double* umax = nullptr) {
   sb = SplineBlob();
   int rows = 0, btot = 0, ktot = 0, urows = 0;
   std::vector<std::vector<int>> starts(ntab);
@@ -602,13 +620,17 @@ static void build_spline_blob(int ntab, const float* const* rho2, const float* c
       st.inv_w = (float)(st.nb / span);
       spline_buckets(k, nt, st.nb, st.r0, st.inv_w, margin_of(st.nb), starts[t]);
     }
-    st.uni = (want_uniform && mode == WFL_MODE_PHYSICAL && spline_uniform(k, nt, st)) ? 1 : 0;
+    st.uni = (umax && mode == WFL_MODE_PHYSICAL && spline_uniform(k, nt, st)) ? 1 : 0;
     st.ucf = -1;
-    st.ukn = -1;
     if (st.uni) {
-      st.ucf = urows;
-      st.ukn = 4 * (urows + nt + 1);
-      urows += nt + 1 + (nt + 1 + 3) / 4;
+      const double need = std::max(spline_uniform_rows(st, umax[t]), (double)(st.ulo + nt + 1));
+      if (need <= (double)(kUniRows - urows)) {
+        st.ucf = urows;
+        st.urows = (int)need;
+        urows += st.urows;
+      } else {
+        st.uni = 0;
+      }
     }
     st.cf = rows;
     rows += nt + 1;
@@ -619,10 +641,11 @@ static void build_spline_blob(int ntab, const float* const* rho2, const float* c
     sb.tab.push_back(st);
   }
   const size_t kn_f4 = ((size_t)ktot + 3) / 4;
-  sb.data.assign((size_t)urows + rows + btot + kn_f4 + 1, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
-  sb.uni_f4 = urows;
+  const int uni_f4 = urows > 0 ? kUniRows + urows : 0;
+  sb.data.assign((size_t)uni_f4 + rows + btot + kn_f4 + 1, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+  sb.uni_f4 = uni_f4;
   sb.all_uni = ntab > 0;
-  float4* d = sb.data.data() + urows;      // the exact-lookup part follows the uniform rows
+  float4* d = sb.data.data() + uni_f4;     // the exact-lookup part follows the uniform rows
   float* kn = (float*)(d + rows + btot);
   const float inf = __builtin_inff();
   for (int t = 0; t < ntab; ++t) {
@@ -639,10 +662,11 @@ static void build_spline_blob(int ntab, const float* const* rho2, const float* c
     sb.all_uni = sb.all_uni && st.uni;
     if (st.uni) {
       float4* u = sb.data.data() + st.ucf;
-      float* ul = (float*)sb.data.data() + st.ukn;
-      for (int i = 0; i <= nt; ++i) {
-        u[i] = d[st.cf + i];
-        ul[i] = i == 0 ? 0.0f : k[i - 1];
+      float4* ul = sb.data.data() + kUniRows + st.ucf;
+      for (int i = 0; i < st.urows; ++i) {
+        const int r = std::min(std::max(i - st.ulo, 0), nt);   // row of the exact lookup
+        u[i] = d[st.cf + r];
+        ul[i].x = r == 0 ? 0.0f : k[r - 1];
       }
     }
     for (int i = 0; i < nt; ++i) kn[st.kn + i] = k[i];
@@ -652,10 +676,26 @@ static void build_spline_blob(int ntab, const float* const* rho2, const float* c
       memcpy(&x, &s0, sizeof x);
       d[rows + st.rec + b] = make_float4(x, k[s0], s0 + 1 <= nt - 2 ? k[s0 + 1] : inf, s0 + 2 <= nt - 2 ? k[s0 + 2] : inf);
     }
-    st.cf += urows;                        // float4 offsets in the blob
-    st.rec += urows + rows;
-    st.kn += 4 * (urows + rows + btot);    // float offset in the blob
+    st.cf += uni_f4;                       // float4 offsets in the blob
+    st.rec += uni_f4 + rows;
+    st.kn += 4 * (uni_f4 + rows + btot);   // float offset in the blob
   }
+}
+
+// Largest rho^2 that a node of a tuple can reach: the farthest tube of its
+// image window (one image more on each side than the kernels sum, against
+// rounding of the window's centre) plus sqrt(T) times the ensemble's
+// largest loop radius (a midpoint lies within the radius of its ends).
+static double node_u_bound(const wfl_geometry& G, double cx, double cy, double rtT, float rmax) {
+  double ax = fabs(cx), ay = fabs(cy);
+  if (G.lattice > 0.0) {
+    const double L = G.lattice, M = (double)G.images + 1.0;
+    const double m0 = floor(cx / L + 0.5), n0 = floor(cy / L + 0.5);
+    ax = std::max(fabs(cx - (m0 - M) * L), fabs(cx - (m0 + M) * L));
+    ay = std::max(fabs(cy - (n0 - M) * L), fabs(cy - (n0 + M) * L));
+  }
+  const double r = sqrt(ax * ax + ay * ay) + rtT * (double)rmax;
+  return r * r;
 }
 
 // Largest spline blob staged in LDS (spline-only launches whose tables are all fast).
@@ -672,10 +712,10 @@ static void fill_geom(const wfl_geometry& g, const SplineBlob* sb, bool scaled, 
     d.spline_n = t.n; d.spline_nb = t.nb; d.spline_fast = t.fast;
     d.spline_r0 = t.r0; d.spline_inv_w = t.inv_w;
     d.spline_k0 = t.k0; d.spline_kl = t.kl;
-    d.spline_ucf = t.ucf; d.spline_ukn = t.ukn;
+    d.spline_ucf = t.ucf;
     d.spline_uinv_h = t.uinv_h; d.spline_uoff = t.uoff;
   } else {
-    d.spline_ucf = -1; d.spline_ukn = -1;
+    d.spline_ucf = -1;
   }
   d.Ff = (float)g.F;       d.F = d.Ff;
   d.l2 = (sg != 1.0) ? WFL_LOG2E / (sg * sg) : (double)(float)g.l2;
@@ -990,7 +1030,16 @@ static int plan_create_impl(wfl_plan** out, const wfl_batch* b, wfl_loops* L, ch
     for (int s = 0; s < b->nspline; ++s) {
       kp.push_back(b->splines[s].rho2); cp.push_back(b->splines[s].coefs); np.push_back(b->splines[s].n);
     }
-    build_spline_blob(b->nspline, kp.data(), cp.data(), np.data(), b->mode, sblob, b->precision == WFL_MIXED);
+    // the uniform-grid rows reach the largest u of each table's tuples
+    std::vector<double> umax((size_t)std::max(b->nspline, 1), 0.0);
+    for (int t = 0; t < b->ntup; ++t) {
+      const wfl_geometry& G = b->geoms[b->tup_geom[t]];
+      if (G.kind != WFL_SPLINE) continue;
+      const int tab = b->nspline == 1 ? 0 : G.spline_table;
+      umax[tab] = std::max(umax[tab], node_u_bound(G, b->tup_cx[t], b->tup_cy[t], sqrt(b->tup_T[t]), rmax));
+    }
+    build_spline_blob(b->nspline, kp.data(), cp.data(), np.data(), b->mode, sblob,
+                      b->precision == WFL_MIXED ? umax.data() : nullptr);
   }
   std::vector<DevGeom> geoms(b->ngeom);
   p->prof_gauss = 1;
@@ -1642,7 +1691,14 @@ extern "C" int wfl_internal_ev_launch(const void* dev_aos, int Nl, int N, const 
       if (!(kn[i] > kn[i - 1])) { wfl_set_error("EV: spline knots not increasing at %d", i); return WFL_ERR_ARG; }
     const float* kp = kn.data();
     const float* cp = cf.data();
-    build_spline_blob(1, &kp, &cp, &nspline, mode, sb, precision == WFL_MIXED);
+    double umax = 0.0;
+    if (precision == WFL_MIXED) {
+      float rmax = 0.0f;
+      rc = loops_rmax(g_ev.L, &rmax);
+      if (rc) return rc;
+      umax = node_u_bound(gg, (double)cx, (double)cy, (double)rtT, rmax);
+    }
+    build_spline_blob(1, &kp, &cp, &nspline, mode, sb, precision == WFL_MIXED ? &umax : nullptr);
     if (sb.data.size() > g_ev.sblob_cap) {
       if (g_ev.d_sblob) (void)hipFree(g_ev.d_sblob);
       g_ev.d_sblob = nullptr; g_ev.sblob_cap = 0;

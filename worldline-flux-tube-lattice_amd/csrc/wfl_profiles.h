@@ -23,6 +23,12 @@ enum { P_ANY = 99 };
 // Kernel-only profile: a spline whose knots lie on a uniform grid in
 // r = sqrt(rho^2), evaluated in FP32 physical mode by direct indexing.
 enum { P_SPLINE_U = 98 };
+// Uniform-grid spline rows in LDS: the rows of every table from byte 0 (at
+// most kUniRows rows of 16 B), and the left knot of row i in the first float
+// of the 16 B slot kUniKnotOff bytes after the row, so that one address
+// (and two immediate offsets) serves both reads.
+constexpr int kUniRows = 1024;
+constexpr int kUniKnotOff = 16 * kUniRows;
 
 // One geometry as the kernels see it.  Every double field holds a value
 // that is exactly representable in FP32.
@@ -40,8 +46,8 @@ struct DevGeom {
   int spline_n, spline_nb, spline_fast;             // knots, buckets, 1: every interval within 2 of its bucket start
   float spline_r0, spline_inv_w;                    // first bucket edge and 1/width in sqrt(rho^2)
   float spline_k0, spline_kl;                       // first and last knot
-  int spline_ucf, spline_ukn;                       // uniform-grid rows (float4 offset, -1: none), their left knots (float offset)
-  float spline_uinv_h, spline_uoff;                 // row of u: trunc(min(sqrt(u) uinv_h + uoff, n))
+  int spline_ucf;                                   // uniform-grid rows (float4 offset, -1: none)
+  float spline_uinv_h, spline_uoff;                 // row of u: trunc(sqrt(u) uinv_h + uoff)
   double F, l2, inv_l2, a, lmin, q1, q2, lattice;   // effective values (FP64 kernels)
   float Ff, l2f, inv_l2f, af, lminf, q1f, q2f;      // FP32 generic-profile values
   float latticef;                                   // lattice spacing in FP32 kernel units
@@ -320,8 +326,7 @@ struct SplineRef {
   const float* kn;        // knots rho^2
   int n, nb, fast;
   float r0, inv_w, k0, kl;
-  const float4* ucf;      // uniform-grid rows
-  const float* ukn;       // their left knots
+  const float4* ucf;      // uniform-grid rows (their knots kUniKnotOff bytes on)
   float uinv_h, uoff;
 };
 
@@ -338,29 +343,33 @@ __device__ __forceinline__ SplineRef spline_ref(const float4* blob, const DevGeo
   s.k0 = g.spline_k0;
   s.kl = g.spline_kl;
   s.ucf = blob + (g.spline_ucf > 0 ? g.spline_ucf : 0);
-  s.ukn = (const float*)blob + (g.spline_ukn > 0 ? g.spline_ukn : 0);
   s.uinv_h = g.spline_uinv_h;
   s.uoff = g.spline_uoff;
   return s;
 }
 
 // Uniform-grid spline, FP32 physical mode: g(u) = f(u)/u and f'(u) from one
-// row in LDS, with no search.  The knots lie on r_j = r0 + j h (checked on
-// the host to 1e-4 h), so row i = trunc((sqrt(u) - r0)/h) + 1, clamped to
-// [0, n], is interval i - 1; row 0 is the line s u below the first knot and
-// row n the constant past the last.
-// Row i holds the coefficients (c0, c1, c2, c3) of the exact lookup's row i
-// (16 B, dense) and, in an array of its own, the row's left knot (0 for
-// row 0, k_{n-1} for row n).  Near a knot, rounding of sqrt(u) may pick
-// the neighbouring interval: interior intervals join with two continuous
-// derivatives and the line with a continuous value, so the difference is at
-// the FP32 rounding level.  u > 0 always holds here (the FP32 kernels add
-// 1e-30).
-__device__ __forceinline__ float spline_u_g(float u, const SplineRef& sp, int n, bool need_fp, float& fp) {
-  const float t = __builtin_amdgcn_fmed3f(fmaf(__builtin_amdgcn_sqrtf(u), sp.uinv_h, sp.uoff), 0.0f, (float)n);
-  const int i = (int)t;
-  const float4 c = sp.ucf[i];
-  const float dx = u - sp.ukn[i];
+// row in LDS, with no search and no clamp.  The knots lie on
+// r_j = r0 + j h (checked on the host to 1e-4 h), so row
+// i = trunc(sqrt(u) uinv_h + uoff) is interval i - 1 - p (p rows below the
+// first knot's row keep uoff >= 0, so the argument of trunc is above -1);
+// rows 0..p are the line s u below the first knot, and the rows from
+// n + p on the constant past the last knot, as many as the largest u that the
+// plan's tuples can reach needs (the host bounds it from the loops' largest
+// radius), so every index stays in the table.
+// Row i holds the coefficients (c0, c1, c2, c3) of the exact lookup's row
+// (16 B, dense) and, kUniKnotOff bytes on, the row's left knot (0 below
+// the first knot, k_{n-1} past the last).  Near a knot, rounding of sqrt(u)
+// may pick the neighbouring interval: interior intervals join with two
+// continuous derivatives and the line with a continuous value, so the
+// difference is at the FP32 rounding level.  u > 0 always holds here (the
+// FP32 kernels add 1e-30).
+__device__ __forceinline__ float spline_u_g(float u, const SplineRef& sp, bool need_fp, float& fp) {
+  const float t = fmaf(__builtin_amdgcn_sqrtf(u), sp.uinv_h, sp.uoff);
+  const float4* row = sp.ucf + (unsigned)t;
+  const float4 c = row[0];
+  const float k = ((const float*)row)[kUniKnotOff / 4];
+  const float dx = u - k;
   const float f = fmaf(dx, fmaf(dx, fmaf(dx, c.w, c.z), c.y), c.x);
   fp = need_fp ? fmaf(dx, fmaf(dx, 3.0f * c.w, 2.0f * c.z), c.y) : 0.0f;
   return f * __builtin_amdgcn_rcpf(u);
@@ -464,7 +473,7 @@ __device__ void profile_eval(int kind, const GP<R>& g, R u, int mode, bool need_
 template <class R, int PROF>
 __device__ __forceinline__ R generic_g(int kind_rt, const GP<R>& g, R u, int mode, bool need_fp,
                                        const SplineRef& sp, R& fp) {
-  if constexpr (PROF == P_SPLINE_U && sizeof(R) == 4) return spline_u_g(u, sp, sp.n, need_fp, fp);
+  if constexpr (PROF == P_SPLINE_U && sizeof(R) == 4) return spline_u_g(u, sp, need_fp, fp);
   const int kind = PROF == P_ANY ? kind_rt : PROF;
   R f;
   if (mode == M_PHYSICAL) {

@@ -1140,7 +1140,9 @@ static int plan_create_impl(wfl_plan** out, const wfl_batch* b, wfl_loops* L, ch
   p->lg_images = lg_images;
   p->ntasks = p->ntasks1 = (int)plan_task_list(p, 1).size();
   // mapping (autotune heuristic: lane-per-tuple needs enough lane groups)
-  // (mapping 3 measured no faster than 1 on the bench workload: it is opt-in or autotuned only)
+  // (mapping 3 is opt-in or autotuned only: its per-loop buffer takes 48 B per
+  // tuple and loop, 13.6 GB on the 55296-tuple, 5120-loop spinor workload,
+  // where it runs 630 against 648 ms for mapping 1)
   int mapping = b->mapping;
   int nloc = 0;
   for (const DevTuple& t : p->tups) nloc += t.loc;

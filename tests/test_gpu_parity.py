@@ -318,6 +318,29 @@ def test_spline_uniform_grid_rows_past_the_table(gpu, orc, loops, spinor):
         assert e1 < 2e-5 and e2 < 1e-5
 
 
+@pytest.mark.parametrize("spinor", [0, 1])
+def test_spline_uniform_grid_on_a_lattice(gpu, orc, loops, spinor):
+    """A uniform-grid spline tube on a square lattice (spacing 1.5, 3 x 3
+    image window): the rows past the last knot must reach the farthest image
+    tube of the window plus the loops' extent, which the plan's bound takes
+    from the window; x_cm off the axes inside the cell.  Mixed physical mode
+    (the direct-index kernels) and FP64 (the exact search), against the
+    oracle."""
+    knots, coefs = gaussian_spline(gpu, n=128, rho_max=6.0, l2=0.25)
+    geoms = [gpu.Geometry(kind=gpu.SPLINE, F=1.0, l2=0.25, a=1.0, lattice=1.5, images=1, spline_table=0)]
+    og = [orc.Geom(orc.SPLINE, F=1.0, l2=0.25, tubedist=1.0, lattice=1.5, images=1, knots=knots, coefs=coefs)]
+    tg, tcx, tcy, tT = tuples([0.1, 0.6, 1.2], [0.01, 0.2, 1.5])
+    tcy = 0.4 * tcx + 0.05
+    for prec, bars in ((gpu.MIXED, (2e-5, 1e-5)), (gpu.FP64, (1e-10, 1e-10))):
+        seen = []
+        gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, 1, 2, spinor, prec, 1, (tg, tcx, tcy, tT),
+                                    splines=[(knots, coefs)], on_plan=lambda p: seen.append(p.uniform_splines))
+        e1, e2 = relerr(pl, rpl), relerr(gs, rgs)
+        print(f"  precision {prec}: uniform {seen[0]} perloop {e1:.2e} gsum {e2:.2e}")
+        assert seen == [1 if prec == gpu.MIXED else 0]
+        assert e1 < bars[0] and e2 < bars[1]
+
+
 @pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 5e-5)])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_spline_irregular_knots_and_mixed_profiles(gpu, orc, loops, prec, tol, mode):

@@ -476,3 +476,42 @@ def test_two_level_estimator(wfl, orc):
         got = wfl.leff_groups(geom, 0, T, 0.3, 0.0, gs, np.array([4, 4]), 0, est)
         ref = orc.leff_groups(og, 0, T, 0.3, 0.0, gs, np.array([4, 4]), 0, est)
         assert np.allclose(got, ref, rtol=1e-12)
+
+
+# analysis-scripts/EAscffvsconst.sage:63-74: the leading-order derivative
+# expansion (locally constant field) of L_eff for the fixflux profile
+# (F = 1, tube l = 0.3, tubedist a = 1, lmin = 0.1, q1 = 0.443991,
+# q2 = 0.0742478), -(1/2 pi) int_0^100 e^-T/T^3 (eBT/sinh(eBT) - 1 +
+# (eBT)^2/6) dT, tabulated by the script on rho = 0.005, 0.010, ... (its
+# first 30 entries; past rho = l/2 only the background field remains).
+FIXFLUX_LCF_TABLE = [
+    -3293.28555160, -3268.32687001, -3227.08914982, -3169.55070950, -3095.55024139,
+    -3005.60011341, -2899.58309952, -2778.08130014, -2641.45440100, -2490.31314204,
+    -2325.39478220, -2147.84432326, -1958.80989950, -1760.12013070, -1553.99342531,
+    -1343.23209559, -1131.33125294, -922.575232475, -722.092884924, -535.815812501,
+    -370.244365454, -231.872091647, -126.085122294, -55.3961420686, -17.2828780744,
+    -2.99638285276, -0.195028060671, -0.0101282156775, -0.00610544135068, -0.00610357265228]
+
+
+def test_fixflux_field_and_exact_reproduce_the_reference_lcf_table(wfl):
+    """The library's fixflux field (MagField, with the script's q1, q2) and
+    its constant-field closed form (Exact, scalar) reproduce the reference
+    analysis script's locally-constant-field table: the T-integral of
+    Exact(T, B(rho)) over (0, 100], times -1/(2 pi), at each tabulated rho,
+    within 5e-5 (the script's own numerical integration of the sharply
+    peaked integrand at the tube centre, eB T ~ 1 at T ~ 0.005, is good to
+    a few 1e-5 there and to ~1e-6 elsewhere).  A wrong field normalisation,
+    core or background term, or a wrong sign or term in Exact fails it."""
+    from scipy import integrate
+    geom = wfl.Geometry(kind=wfl.FIXFLUX, F=1.0, l2=0.09, a=1.0, lam_min=0.1, fix_q1=0.443991, fix_q2=0.0742478)
+    wfl.set_profile(geom)
+    p = wfl.Wparams()
+    p.F, p.l2 = 1.0, 0.09
+    for i, want in enumerate(FIXFLUX_LCF_TABLE):
+        rho = 0.005 * (i + 1)
+        B = wfl.lib.MagField(rho * rho, C.byref(p))
+        # (no break points: near T = 0 the closed form's 1 - 1 + (eBT)^2/6 cancels,
+        # and an adaptive rule that is made to sample there reads the noise)
+        val = -integrate.quad(lambda T: wfl.lib.Exact(T, B, 0), 0.0, 100.0, limit=800, epsabs=0.0,
+                              epsrel=1e-11)[0] / (2.0 * np.pi)
+        assert abs(val - want) <= 5e-5 * abs(want), (rho, val, want)

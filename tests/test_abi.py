@@ -515,3 +515,18 @@ def test_fixflux_field_and_exact_reproduce_the_reference_lcf_table(wfl):
         val = -integrate.quad(lambda T: wfl.lib.Exact(T, B, 0), 0.0, 100.0, limit=800, epsabs=0.0,
                               epsrel=1e-11)[0] / (2.0 * np.pi)
         assert abs(val - want) <= 5e-5 * abs(want), (rho, val, want)
+
+
+def test_exact_reproduces_the_reference_scripts_constant_field_values(wfl):
+    """The reference's analysis scripts plot the constant-field L_eff at
+    eB = m^2 = 1 as `exact`: 0.0139688 for scalar QED
+    (analysis-scripts/Leffvsrho.sage:17) and -0.0164599 for spinor QED
+    (Leffvsrho.F1l1.sage:30), the proper-time integrals int_0^inf of the
+    closed forms.  The library's Exact (Integrand.c:20-37) integrated over
+    T at B = 1/e gives both to the scripts' 6 printed digits (half a unit
+    of the last digit, 5e-8, plus the quadrature's error)."""
+    from scipy import integrate
+    e = 0.30282212
+    for fermion, want in ((0, 0.0139688), (1, -0.0164599)):
+        val = integrate.quad(lambda T: wfl.lib.Exact(T, 1.0 / e, fermion), 0.0, np.inf, limit=500, epsrel=1e-10)[0]
+        assert abs(val - want) <= 5.5e-8, (fermion, val, want)

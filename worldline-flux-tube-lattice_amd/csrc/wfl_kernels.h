@@ -228,24 +228,45 @@ __device__ __forceinline__ bool guard600(double v) {
   return fabs(trunc(v)) > 600.0;
 }
 __device__ __forceinline__ void wminus1(int mode, bool spinor, const double sc[3], const double fe[3], double wm[3]) {
-  double yz = sc[1] + sc[2];
-  double a[3] = {0.5 * sc[0] + 0.25 * yz, sc[0], 0.5 * yz};
+  // The three estimates take the angles a1 = sc0, a2 = (sc1 + sc2)/2 and
+  // their mean a0 = (a1 + a2)/2 (likewise b from fe), so every half-angle
+  // sine follows from one sincos of a1/4 and one of a2/4 by the double-angle
+  // and addition formulas (bounded terms: absolute error of a few ulp, as
+  // for the direct evaluation), and the hyperbolic values from expm1 of
+  // b1/4, b2/4 and b0/2 (b0/2 directly: b1 and b2 of opposite signs would
+  // cancel in an addition formula): five transcendental calls for nine.
+  double s1, c1, s2, c2;
+  sincos(0.25 * sc[0], &s1, &c1);
+  sincos(0.125 * (sc[1] + sc[2]), &s2, &c2);
+  const double sh[3] = {fma(s1, c2, c1 * s2), 2.0 * s1 * c1, 2.0 * s2 * c2};   // sin(a_k / 2)
   if (!spinor) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) { double s = sin(0.5 * a[k]); wm[k] = -2.0 * s * s; }
+    for (int k = 0; k < 3; ++k) wm[k] = -2.0 * sh[k] * sh[k];
     return;
   }
   double fyz = fe[1] + fe[2];
-  double b[3] = {0.5 * fe[0] + 0.25 * fyz, fe[0], 0.5 * fyz};
   if (mode == M_REFERENCE && (guard600(fyz) || guard600(fe[0]))) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) wm[k] = cos(a[k]) * 1.0e10 - 1.0;
+    for (int k = 0; k < 3; ++k) wm[k] = fma(-2.0 * sh[k], sh[k], 1.0) * 1.0e10 - 1.0;   // cos(a_k) 1e10 - 1
     return;
   }
+  // sinh and cosh of p = b1/4 and q = b2/4 from e = expm1(|p|) >= 0: with
+  // E = e + 1 >= 1, sinh|p| = e (1 + 1/E)/2 (no cancellation for small
+  // arguments, and inf, not NaN, when E overflows) and cosh = (E + 1/E)/2;
+  // sinh takes the sign of p
+  const double p = 0.25 * fe[0], q = 0.125 * fyz;
+  const double ep = expm1(fabs(p)), eq = expm1(fabs(q));
+  const double ip = 1.0 / (ep + 1.0), iq = 1.0 / (eq + 1.0);
+  const double shp = copysign(0.5 * ep * (1.0 + ip), p), chp = 0.5 * ((ep + 1.0) + ip);
+  const double shq = copysign(0.5 * eq * (1.0 + iq), q), chq = 0.5 * ((eq + 1.0) + iq);
+  const double r = p + q;                                   // b0 / 2
+  const double er = expm1(fabs(r));
+  const double shr = copysign(0.5 * er * (1.0 + 1.0 / (er + 1.0)), r);
+  const double hh[3] = {shr, 2.0 * shp * chp, 2.0 * shq * chq};   // sinh(b_k / 2)
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    double s = sin(0.5 * a[k]), sh = sinh(0.5 * b[k]);
-    wm[k] = 2.0 * sh * sh - 2.0 * s * s * cosh(b[k]);
+    const double h2 = 2.0 * hh[k] * hh[k];          // cosh(b_k) - 1
+    wm[k] = h2 - 2.0 * sh[k] * sh[k] * (1.0 + h2);   // 2 sinh^2(b/2) - 2 sin^2(a/2) cosh(b)
   }
 }
 

@@ -510,17 +510,18 @@ static double unit_scale(const wfl_geometry& g, bool scaled) {
 // u is at most s + 2 for every bucket; nb is the smallest of n/2, n, 2n, 4n,
 // 8n (at most 16384) for which it is, else 2n with the scan.
 // Uniform-grid rows (FP32 physical mode, tables whose knots lie on
-// r_j = r0 + j h), at the front of the blob: kUniRows float4 of rows (each
-// table's p copies of the row below the first knot, its n + 1 rows, and
-// copies of the row past the last knot up to the largest u its tuples
-// reach), then one float4 per row holding that row's left knot in .x.
+// r_j = (m + j) h, m an integer), at the front of the blob: kUniRows float4
+// of rows (each table's guard row, a copy of its row 0, then m copies of the
+// row below the first knot, its n - 1 intervals, and copies of the row past
+// the last knot up to the largest u its tuples reach), then one float4 per
+// row holding that row's left knot in .x.
 // uni_f4 is the size of that prefix, the part the uniform-grid kernels
 // stage in LDS.  A table whose padded rows do not fit is not uniform.
 struct SplineTab {
   int cf, rec, kn, n, nb, fast;
   float r0, inv_w, k0, kl;
-  int uni, ucf, ulo, urows;   // uniform rows: offset, rows below the first knot's, rows in all
-  float uinv_h, uoff;
+  int uni, ucf, ulo, urows;   // uniform rows: offset (after the guard row), last row below the first knot, rows
+  float uinv_h;
 };
 struct SplineBlob {
   std::vector<float4> data;
@@ -529,36 +530,34 @@ struct SplineBlob {
   bool all_uni = false;  // every table has them
 };
 
-// Knots on a uniform grid in r = sqrt(rho^2): with inv_h = fl32(1/h),
-// p = the rows kept below the first knot's row (so that off >= 0) and
-// off = fl32(1 + p - r0 inv_h), every knot k_j has sqrt(k_j) inv_h + off
-// within 1e-4 of j + 1 + p.
+// Knots on a uniform grid in r = sqrt(rho^2) that starts on the grid:
+// r_j = (m + j) h with an integer m >= 0 (periodic.c:129-151 puts the first
+// knot at rho = h, m = 1).  With inv_h = fl32(1/h) every knot k_j has
+// sqrt(k_j) inv_h within 1e-4 of m + j, so the kernel's row
+// floor(sqrt(u) inv_h) is interval row - m; p = m - 1 is the last row
+// below the first knot (-1 when the first knot is rho = 0).
 static bool spline_uniform(const float* k, int n, SplineTab& st) {
   if (n < 3 || !(k[0] >= 0.0f)) return false;
   const double r0 = sqrt((double)k[0]), r1 = sqrt((double)k[n - 1]);
   const double h = (r1 - r0) / (double)(n - 1);
   if (!(h > 0.0)) return false;
   const float inv_h = (float)(1.0 / h);
-  const double off0 = 1.0 - r0 * (double)inv_h;
-  if (!(off0 > -(double)kUniRows)) return false;
-  const int p = off0 < 0.0 ? (int)ceil(-off0) : 0;
-  const float off = (float)(off0 + p);
-  if (!(off >= 0.0f)) return false;
+  const double m = nearbyint(r0 * (double)inv_h);
+  if (!(m >= 0.0 && m < (double)kUniRows)) return false;
   for (int j = 0; j < n; ++j) {
-    const double t = sqrt((double)k[j]) * (double)inv_h + (double)off;
-    if (!(fabs(t - (double)(j + 1 + p)) <= 1e-4)) return false;
+    const double t = sqrt((double)k[j]) * (double)inv_h;
+    if (!(fabs(t - (m + (double)j)) <= 1e-4)) return false;
   }
   st.uinv_h = inv_h;
-  st.uoff = off;
-  st.ulo = p;
+  st.ulo = (int)m - 1;
   return true;
 }
 
 // Rows a uniform table needs for nodes up to u = umax: the kernel's row
-// trunc(sqrt(u) inv_h + off), FP32-rounded, plus two rows of margin.
+// floor(sqrt(u) inv_h), FP32-rounded, plus two rows of margin.
 static double spline_uniform_rows(const SplineTab& st, double umax) {
   if (!(umax >= 0.0) || !std::isfinite(umax)) return HUGE_VAL;
-  const double t = sqrt(umax * (1.0 + 1e-5)) * (double)st.uinv_h * (1.0 + 1e-6) + (double)st.uoff;
+  const double t = sqrt(umax * (1.0 + 1e-5)) * (double)st.uinv_h * (1.0 + 1e-6);
   return floor(t) + 3.0;
 }
 
@@ -624,10 +623,10 @@ static void build_spline_blob(int ntab, const float* const* rho2, const float* c
     st.ucf = -1;
     if (st.uni) {
       const double need = std::max(spline_uniform_rows(st, umax[t]), (double)(st.ulo + nt + 1));
-      if (need <= (double)(kUniRows - urows)) {
-        st.ucf = urows;
+      if (need + 1.0 <= (double)(kUniRows - urows)) {   // the rows and the guard row before them
+        st.ucf = urows + 1;
         st.urows = (int)need;
-        urows += st.urows;
+        urows += st.urows + 1;
       } else {
         st.uni = 0;
       }
@@ -668,6 +667,8 @@ static void build_spline_blob(int ntab, const float* const* rho2, const float* c
         u[i] = d[st.cf + r];
         ul[i].x = r == 0 ? 0.0f : k[r - 1];
       }
+      u[-1] = u[0];     // guard row: the kernel's row for sqrt(u) inv_h < 1/4
+      ul[-1] = ul[0];
     }
     for (int i = 0; i < nt; ++i) kn[st.kn + i] = k[i];
     for (int b = 0; b < st.nb; ++b) {
@@ -713,7 +714,7 @@ static void fill_geom(const wfl_geometry& g, const SplineBlob* sb, bool scaled, 
     d.spline_r0 = t.r0; d.spline_inv_w = t.inv_w;
     d.spline_k0 = t.k0; d.spline_kl = t.kl;
     d.spline_ucf = t.ucf;
-    d.spline_uinv_h = t.uinv_h; d.spline_uoff = t.uoff;
+    d.spline_uinv_h = t.uinv_h;
   } else {
     d.spline_ucf = -1;
   }

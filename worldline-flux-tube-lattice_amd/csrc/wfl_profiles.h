@@ -47,7 +47,7 @@ struct DevGeom {
   float spline_r0, spline_inv_w;                    // first bucket edge and 1/width in sqrt(rho^2)
   float spline_k0, spline_kl;                       // first and last knot
   int spline_ucf;                                   // uniform-grid rows (float4 offset, -1: none)
-  float spline_uinv_h, spline_uoff;                 // row of u: trunc(sqrt(u) uinv_h + uoff)
+  float spline_uinv_h;                              // row of u: floor(sqrt(u) uinv_h)
   double F, l2, inv_l2, a, lmin, q1, q2, lattice;   // effective values (FP64 kernels)
   float Ff, l2f, inv_l2f, af, lminf, q1f, q2f;      // FP32 generic-profile values
   float latticef;                                   // lattice spacing in FP32 kernel units
@@ -327,7 +327,8 @@ struct SplineRef {
   int n, nb, fast;
   float r0, inv_w, k0, kl;
   const float4* ucf;      // uniform-grid rows (their knots kUniKnotOff bytes on)
-  float uinv_h, uoff;
+  float uinv_h;
+  unsigned ubias;         // LDS byte address of ucf less 0x4B000000 * 16 (mod 2^32)
 };
 
 __device__ __forceinline__ SplineRef spline_ref(const float4* blob, const DevGeom& g) {
@@ -344,19 +345,29 @@ __device__ __forceinline__ SplineRef spline_ref(const float4* blob, const DevGeo
   s.kl = g.spline_kl;
   s.ucf = blob + (g.spline_ucf > 0 ? g.spline_ucf : 0);
   s.uinv_h = g.spline_uinv_h;
-  s.uoff = g.spline_uoff;
+  // only the uniform-grid kernels read it, and they stage the rows in LDS;
+  // the empty asm keeps the bias out of the compiler's reach, so that each
+  // node adds the shifted row index to it in one instruction
+  s.ubias = (unsigned)(uintptr_t)(const __attribute__((address_space(3))) float4*)s.ucf + 0x50000000u;
+  asm("" : "+v"(s.ubias));
   return s;
 }
 
 // Uniform-grid spline, FP32 physical mode: g(u) = f(u)/u and f'(u) from one
 // row in LDS, with no search and no clamp.  The knots lie on
-// r_j = r0 + j h (checked on the host to 1e-4 h), so row
-// i = trunc(sqrt(u) uinv_h + uoff) is interval i - 1 - p (p rows below the
-// first knot's row keep uoff >= 0, so the argument of trunc is above -1);
-// rows 0..p are the line s u below the first knot, and the rows from
-// n + p on the constant past the last knot, as many as the largest u that the
-// plan's tuples can reach needs (the host bounds it from the loops' largest
-// radius), so every index stays in the table.
+// r_j = (m + j) h with an integer m >= 0 (checked on the host to 1e-4 h), so
+// row i = floor(sqrt(u) uinv_h) is interval i - m; rows 0..m-1 are the line
+// s u below the first knot, and the rows from n - 1 + m on the constant past
+// the last knot, as many as the largest u that the plan's tuples can reach
+// needs (the host bounds it from the loops' largest radius), so every index
+// stays in the table.
+// The floor comes from the rounding of one FMA: y + (2^23 - 1/2) with
+// y = sqrt(u) uinv_h >= 1/4 lies in [2^23, 2^24), where FP32 steps by 1, so
+// it rounds to 2^23 + floor(y) (round to nearest of y - 1/2; a tie, y on a
+// knot, may give the row below, which meets the row above there), and the
+// row's byte offset is the result's bit pattern times 16 less
+// 0x4B000000 * 16; y < 1/4 rounds to 2^23 - 1/2, the bit pattern one below,
+// which reads the guard row -1 (a copy of row 0).
 // Row i holds the coefficients (c0, c1, c2, c3) of the exact lookup's row
 // (16 B, dense) and, kUniKnotOff bytes on, the row's left knot (0 below
 // the first knot, k_{n-1} past the last).  Near a knot, rounding of sqrt(u)
@@ -365,8 +376,9 @@ __device__ __forceinline__ SplineRef spline_ref(const float4* blob, const DevGeo
 // difference is at the FP32 rounding level.  u > 0 always holds here (the
 // FP32 kernels add 1e-30).
 __device__ __forceinline__ float spline_u_g(float u, const SplineRef& sp, bool need_fp, float& fp) {
-  const float t = fmaf(__builtin_amdgcn_sqrtf(u), sp.uinv_h, sp.uoff);
-  const float4* row = sp.ucf + (unsigned)t;
+  typedef const __attribute__((address_space(3))) float4 LdsRow;
+  const float t = fmaf(__builtin_amdgcn_sqrtf(u), sp.uinv_h, 8388607.5f);
+  const float4* row = (const float4*)(LdsRow*)(uintptr_t)((__float_as_uint(t) << 4) + sp.ubias);
   const float4 c = row[0];
   const float k = ((const float*)row)[kUniKnotOff / 4];
   const float dx = u - k;

@@ -455,6 +455,59 @@ __device__ __forceinline__ void loop_sums(const R* __restrict__ rl, int N, int M
   }
 }
 
+// Class sums of one loop over the (2M+1)^2 image tubes of a lattice
+// window, points outer and image tubes inner.  In a square window z's x
+// part depends only on the window column and its y part only on the row,
+// so per point each column costs z_x and the midpoint's x (2 VALU), each
+// row z_y, the midpoint's y, their squares and z_y e_x, z_y 4 dw_x (6), and
+// each image tube its two rho^2 FMAs, two nodes, two cross products and the
+// two accumulations, with the same roundings as point_eval.  Physical
+// mode, n = 2, the plain node (the caller has checked that no image tube
+// of the wavefront asks for the accurate Gaussian node).  The FP32 sums
+// over one point's image tubes are added to P and S in FP64 per point.
+template <class R, int PROF, int SPINOR, int M>
+__device__ __forceinline__ void lattice_sums(const R* __restrict__ rl, int N, const R* ccx, const R* ccy,
+                                             const LaneC<R>& lc, int kind, const GP<R>& gp, const SplineRef& sp,
+                                             double P[3], double S[3]) {
+  constexpr int W = 2 * M + 1;
+  const R tiny = sizeof(R) == 4 ? (R)1e-30 : (R)0;   // keeps 1/u finite on the axis (FP32 node)
+  for (int i = 0; i < N; i += 4) {
+    const R* r4 = rl + (size_t)i * REC;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int c = (u & 1) ? 0 : (u == 0 ? 2 : 1);
+      const R* r = r4 + REC * u;
+      const R wx = r[0], wy = r[1], dx4 = r[2], dy4 = r[3], ex = r[4], ey = r[5];
+      R zx[W], mx[W];
+#pragma unroll
+      for (int a = 0; a < W; ++a) {
+        zx[a] = fma(lc.s, wx, ccx[a]);
+        mx[a] = fma(lc.m8, dx4, zx[a]);
+      }
+      R p = (R)0, q = (R)0;
+#pragma unroll
+      for (int b = 0; b < W; ++b) {
+        const R zy = fma(lc.s, wy, ccy[b]);
+        const R my = fma(lc.m8, dy4, zy);
+        const R zy2 = fma(zy, zy, tiny), my2 = fma(my, my, tiny);
+        const R zyex = zy * ex, zydx = zy * dx4;
+#pragma unroll
+        for (int a = 0; a < W; ++a) {
+          R fA, fk;
+          const R gA = node_g<R, PROF, 0>(fma(zx[a], zx[a], zy2), lc, kind, gp, M_PHYSICAL, sp, SPINOR, fA);
+          const R gm = node_g<R, PROF, 0>(fma(mx[a], mx[a], my2), lc, kind, gp, M_PHYSICAL, sp, SPINOR, fk);
+          const R E = fma(zx[a], ey, -zyex);    // z x e
+          const R X = fma(zx[a], dy4, -zydx);   // 4 z x dw
+          p = fma(E, gA, fma(X, gm, p));
+          if (SPINOR) q += fA + (fk + fk);
+        }
+      }
+      P[c] += (double)p;
+      if (SPINOR) S[c] += (double)q;
+    }
+  }
+}
+
 // SPLIT = 1 writes each loop's (Wsscal, Wsferm) to perloop and leaves W - 1
 // and the group sums to k_finalize_groups: the FP64 trigonometry then does
 // not set this kernel's register allocation (more resident wavefronts).
@@ -496,6 +549,24 @@ __device__ __forceinline__ void tuple_task(
   constexpr bool kLocal = MODE == M_AXIS && PROF == P_GAUSSIAN && sizeof(R) == 4;
   const bool loc_group = kLocal && linfo && __builtin_amdgcn_readfirstlane(tp.loc) != 0;
 
+  // FP32, 3 x 3 and 5 x 5 windows, physical mode, n = 2: image tubes inner
+  // (lattice_sums) unless an image tube of the wavefront asks for the
+  // accurate Gaussian node (a choice that depends on the tuples only).  The
+  // FP64 kernels keep the image tubes outer: unrolled over the window, the
+  // software exp and divide of the FP64 node take 256 VGPRs and spill.
+  constexpr bool kLatInner = MODE == M_PHYSICAL && NSUB == 2 && sizeof(R) == 4;
+  bool lat_inner = false;
+  if constexpr (kLatInner) {
+    if (lattice && (M == 1 || M == 2)) {
+      lat_inner = true;
+      if constexpr (PROF == P_GAUSSIAN)
+        for (int ia = -M; ia <= M; ++ia)
+          for (int ib = -M; ib <= M; ++ib)
+            if (image_acc<R>(gm, lc, fma(-(R)(tp.m0 + ia), lat, cx), fma(-(R)(tp.n0 + ib), lat, cy)) != 0)
+              lat_inner = false;
+    }
+  }
+
   double G[3] = {0.0, 0.0, 0.0};
   for (int l = loop0; l < loop0 + nloop; ++l) {
     const R* __restrict__ rl = (const R*)__builtin_assume_aligned(rec + (size_t)l * N * RS, 2 * sizeof(R));
@@ -525,6 +596,21 @@ __device__ __forceinline__ void tuple_task(
           P[2] = fma(a0s, linfo[l].q[2], P[2]);
           done = true;
         }
+      }
+    }
+    if constexpr (kLatInner) {
+      if (lat_inner) {
+        R ccx[5], ccy[5];
+#pragma unroll
+        for (int a = 0; a < 5; ++a) {
+          ccx[a] = fma(-(R)(tp.m0 + a - M), lat, cx);
+          ccy[a] = fma(-(R)(tp.n0 + a - M), lat, cy);
+        }
+        if (M == 1)
+          lattice_sums<R, PROF, SPINOR, 1>(rl, N, ccx, ccy, lc, kind, gp, sp, P, S);
+        else
+          lattice_sums<R, PROF, SPINOR, 2>(rl, N, ccx, ccy, lc, kind, gp, sp, P, S);
+        done = true;
       }
     }
     if (!done)

@@ -510,18 +510,19 @@ static double unit_scale(const wfl_geometry& g, bool scaled) {
 // u is at most s + 2 for every bucket; nb is the smallest of n/2, n, 2n, 4n,
 // 8n (at most 16384) for which it is, else 2n with the scan.
 // Uniform-grid rows (FP32 physical mode, tables whose knots lie on
-// r_j = (m + j) h, m an integer), at the front of the blob: kUniRows float4
-// of rows (each table's guard row, a copy of its row 0, then m copies of the
-// row below the first knot, its n - 1 intervals, and copies of the row past
-// the last knot up to the largest u its tuples reach), then one float4 per
-// row holding that row's left knot in .x.
+// r_j = (m + j) h, m an integer), at the front of the blob: at most kUniRows
+// float4 of rows (each table's guard row with its row 0's polynomial, then m
+// copies of the row below the first knot, its n - 1 intervals, and copies of
+// the row past the last knot up to the largest u its tuples reach), each
+// cubic re-expanded about the point the kernel computes for its row,
+// kappa_i = fl32(i h)^2 (kappa = (h/2)^2 for the guard row).
 // uni_f4 is the size of that prefix, the part the uniform-grid kernels
 // stage in LDS.  A table whose padded rows do not fit is not uniform.
 struct SplineTab {
   int cf, rec, kn, n, nb, fast;
   float r0, inv_w, k0, kl;
   int uni, ucf, ulo, urows;   // uniform rows: offset (after the guard row), last row below the first knot, rows
-  float uinv_h;
+  float uinv_h, uh;
 };
 struct SplineBlob {
   std::vector<float4> data;
@@ -549,6 +550,7 @@ static bool spline_uniform(const float* k, int n, SplineTab& st) {
     if (!(fabs(t - (m + (double)j)) <= 1e-4)) return false;
   }
   st.uinv_h = inv_h;
+  st.uh = (float)h;
   st.ulo = (int)m - 1;
   return true;
 }
@@ -640,7 +642,7 @@ static void build_spline_blob(int ntab, const float* const* rho2, const float* c
     sb.tab.push_back(st);
   }
   const size_t kn_f4 = ((size_t)ktot + 3) / 4;
-  const int uni_f4 = urows > 0 ? kUniRows + urows : 0;
+  const int uni_f4 = urows;
   sb.data.assign((size_t)uni_f4 + rows + btot + kn_f4 + 1, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
   sb.uni_f4 = uni_f4;
   sb.all_uni = ntab > 0;
@@ -660,15 +662,21 @@ static void build_spline_blob(int ntab, const float* const* rho2, const float* c
     }
     sb.all_uni = sb.all_uni && st.uni;
     if (st.uni) {
+      // row i from the exact lookup's row r (expanded about its left knot:
+      // 0 for the line, k[r - 1] otherwise) shifted exactly, in double, to
+      // kappa_i = fl32(i h)^2, what the kernel forms from the row index
+      // (i = -1/2 for the guard row, the kernel's row for sqrt(u) inv_h < 1/4)
       float4* u = sb.data.data() + st.ucf;
-      float4* ul = sb.data.data() + kUniRows + st.ucf;
-      for (int i = 0; i < st.urows; ++i) {
-        const int r = std::min(std::max(i - st.ulo, 0), nt);   // row of the exact lookup
-        u[i] = d[st.cf + r];
-        ul[i].x = r == 0 ? 0.0f : k[r - 1];
+      for (int i = -1; i < st.urows; ++i) {
+        const int r = std::min(std::max(std::max(i, 0) - st.ulo, 0), nt);
+        const float4 cr = d[st.cf + r];
+        const double kr = r == 0 ? 0.0 : (double)k[r - 1];
+        const double ri = (double)(float)((i < 0 ? -0.5 : (double)i) * (double)st.uh);
+        const double e = ri * ri - kr;
+        const double c0 = cr.x, c1 = cr.y, c2 = cr.z, c3 = cr.w;
+        u[i] = make_float4((float)(c0 + e * (c1 + e * (c2 + e * c3))), (float)(c1 + e * (2.0 * c2 + 3.0 * c3 * e)),
+                           (float)(c2 + 3.0 * c3 * e), (float)c3);
       }
-      u[-1] = u[0];     // guard row: the kernel's row for sqrt(u) inv_h < 1/4
-      ul[-1] = ul[0];
     }
     for (int i = 0; i < nt; ++i) kn[st.kn + i] = k[i];
     for (int b = 0; b < st.nb; ++b) {
@@ -715,6 +723,7 @@ static void fill_geom(const wfl_geometry& g, const SplineBlob* sb, bool scaled, 
     d.spline_k0 = t.k0; d.spline_kl = t.kl;
     d.spline_ucf = t.ucf;
     d.spline_uinv_h = t.uinv_h;
+    d.spline_uh = t.uh;
   } else {
     d.spline_ucf = -1;
   }

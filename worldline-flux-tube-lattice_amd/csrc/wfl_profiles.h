@@ -23,12 +23,9 @@ enum { P_ANY = 99 };
 // Kernel-only profile: a spline whose knots lie on a uniform grid in
 // r = sqrt(rho^2), evaluated in FP32 physical mode by direct indexing.
 enum { P_SPLINE_U = 98 };
-// Uniform-grid spline rows in LDS: the rows of every table from byte 0 (at
-// most kUniRows rows of 16 B), and the left knot of row i in the first float
-// of the 16 B slot kUniKnotOff bytes after the row, so that one address
-// (and two immediate offsets) serves both reads.
+// Uniform-grid spline rows in LDS: the rows of every table from byte 0, at
+// most kUniRows rows of 16 B.
 constexpr int kUniRows = 1024;
-constexpr int kUniKnotOff = 16 * kUniRows;
 
 // One geometry as the kernels see it.  Every double field holds a value
 // that is exactly representable in FP32.
@@ -47,7 +44,7 @@ struct DevGeom {
   float spline_r0, spline_inv_w;                    // first bucket edge and 1/width in sqrt(rho^2)
   float spline_k0, spline_kl;                       // first and last knot
   int spline_ucf;                                   // uniform-grid rows (float4 offset, -1: none)
-  float spline_uinv_h;                              // row of u: floor(sqrt(u) uinv_h)
+  float spline_uinv_h, spline_uh;                   // row of u: floor(sqrt(u) uinv_h); h = fl32 grid step
   double F, l2, inv_l2, a, lmin, q1, q2, lattice;   // effective values (FP64 kernels)
   float Ff, l2f, inv_l2f, af, lminf, q1f, q2f;      // FP32 generic-profile values
   float latticef;                                   // lattice spacing in FP32 kernel units
@@ -326,8 +323,8 @@ struct SplineRef {
   const float* kn;        // knots rho^2
   int n, nb, fast;
   float r0, inv_w, k0, kl;
-  const float4* ucf;      // uniform-grid rows (their knots kUniKnotOff bytes on)
-  float uinv_h;
+  const float4* ucf;      // uniform-grid rows
+  float uinv_h, uh, uhoff;  // 1/h, h and -2^23 h (exact)
   unsigned ubias;         // LDS byte address of ucf less 0x4B000000 * 16 (mod 2^32)
 };
 
@@ -345,6 +342,8 @@ __device__ __forceinline__ SplineRef spline_ref(const float4* blob, const DevGeo
   s.kl = g.spline_kl;
   s.ucf = blob + (g.spline_ucf > 0 ? g.spline_ucf : 0);
   s.uinv_h = g.spline_uinv_h;
+  s.uh = g.spline_uh;
+  s.uhoff = -8388608.0f * g.spline_uh;
   // only the uniform-grid kernels read it, and they stage the rows in LDS;
   // the empty asm keeps the bias out of the compiler's reach, so that each
   // node adds the shifted row index to it in one instruction
@@ -363,16 +362,18 @@ __device__ __forceinline__ SplineRef spline_ref(const float4* blob, const DevGeo
 // stays in the table.
 // The floor comes from the rounding of one FMA: y + (2^23 - 1/2) with
 // y = sqrt(u) uinv_h >= 1/4 lies in [2^23, 2^24), where FP32 steps by 1, so
-// it rounds to 2^23 + floor(y) (round to nearest of y - 1/2; a tie, y on a
-// knot, may give the row below, which meets the row above there), and the
-// row's byte offset is the result's bit pattern times 16 less
-// 0x4B000000 * 16; y < 1/4 rounds to 2^23 - 1/2, the bit pattern one below,
-// which reads the guard row -1 (a copy of row 0).
-// Row i holds the coefficients (c0, c1, c2, c3) of the exact lookup's row
-// (16 B, dense) and, kUniKnotOff bytes on, the row's left knot (0 below
-// the first knot, k_{n-1} past the last).  Near a knot, rounding of sqrt(u)
-// may pick the neighbouring interval: interior intervals join with two
-// continuous derivatives and the line with a continuous value, so the
+// it rounds to t = 2^23 + floor(y) (round to nearest of y - 1/2; a tie, y on
+// a knot, may give the row below, which meets the row above there), and the
+// row's byte offset is t's bit pattern times 16 less 0x4B000000 * 16;
+// y < 1/4 rounds to t = 2^23 - 1/2, the bit pattern one below, which reads
+// the guard row -1 (row 0's polynomial).
+// Row i holds the cubic (c0, c1, c2, c3) of the exact lookup's row
+// re-expanded on the host about kappa_i = r_i^2, r_i = fl32(i h) (i = -1/2
+// for the guard row): the kernel forms r_i = fma(t, h, -2^23 h) (the product
+// and the difference exact, one rounding) while the row is read and
+// dx = u - r_i^2 in one FMA, so no knot is read.  Near a knot the FP32
+// sqrt(u) may pick the neighbouring interval: interior intervals join with
+// two continuous derivatives and the line with a continuous value, so the
 // difference is at the FP32 rounding level.  u > 0 always holds here (the
 // FP32 kernels add 1e-30).
 __device__ __forceinline__ float spline_u_g(float u, const SplineRef& sp, bool need_fp, float& fp) {
@@ -380,8 +381,8 @@ __device__ __forceinline__ float spline_u_g(float u, const SplineRef& sp, bool n
   const float t = fmaf(__builtin_amdgcn_sqrtf(u), sp.uinv_h, 8388607.5f);
   const float4* row = (const float4*)(LdsRow*)(uintptr_t)((__float_as_uint(t) << 4) + sp.ubias);
   const float4 c = row[0];
-  const float k = ((const float*)row)[kUniKnotOff / 4];
-  const float dx = u - k;
+  const float rk = fmaf(t, sp.uh, sp.uhoff);
+  const float dx = fmaf(-rk, rk, u);
   const float f = fmaf(dx, fmaf(dx, fmaf(dx, c.w, c.z), c.y), c.x);
   fp = need_fp ? fmaf(dx, fmaf(dx, 3.0f * c.w, 2.0f * c.z), c.y) : 0.0f;
   return f * __builtin_amdgcn_rcpf(u);

@@ -230,12 +230,14 @@ def accuracy(wfl, cfg, gs_mixed, gs_64):
     w_m = 1.0 + (gs_mixed.sum(axis=1) @ c) / np.maximum(cnt, 1.0)
     w_d = 1.0 + (gs_64.sum(axis=1) @ c) / np.maximum(cnt, 1.0)
     res["W_max_rel_err"] = float(np.max(np.abs(w_m - w_d) / np.abs(w_d)))
-    res["W_max_abs_err"] = float(np.max(np.abs(w_m - w_d)))
     eb = np.array([2.0 * g["F"] / g["l2"] if g["F"] <= 1.0 and not g.get("lattice", 0.0) else np.inf
                    for g in cfg["geoms"]])
     moderate = (eb[tg] * tT <= 10.0) & (cnt > 0)
     res["W_rel_err_eBT_le_10"] = (float(np.max(np.abs(w_m - w_d)[moderate] / np.abs(w_d[moderate])))
                                   if moderate.any() else None)
+    # absolute error of <W> - 1 over the same tuples (beyond eB T ~ 10 the
+    # spin factor's e^(eB T) makes absolute figures meaningless)
+    res["W_abs_err_eBT_le_10"] = float(np.max(np.abs(w_m - w_d)[moderate])) if moderate.any() else None
     # geometries whose ensemble estimate is not dominated by noise (the sign problem of strong
     # fields drives the median SEM/|L_eff| to ~1)
     well = [p["action_rel_err"] for p in res["tfunc_policy"]

@@ -270,14 +270,16 @@ def test_spline_uniform_grid_at_knots(gpu, orc, loops, spinor):
 @pytest.mark.parametrize("spinor", [0, 1])
 def test_spline_small_tables(gpu, orc, loops, spinor):
     """The smallest tables the spline set-up builds (3 and 5 knots), a
-    uniform table whose first knot is rho = 0, one whose first knot lies
-    three spacings out (rows below it clamp to the line row), and one with
-    a knot off the grid (the exact search): mixed physical mode, against the
-    oracle."""
+    uniform table whose first knot is rho = 0 (no line rows; its guard row
+    is interval 0), one whose first knot lies three spacings out (rows below
+    it repeat the line row), one evenly spaced but starting 3.17 spacings
+    out (off the grid that the direct index needs: the exact search), and one
+    with a knot off the grid (the exact search): mixed physical mode,
+    against the oracle."""
     off_grid = np.arange(8) * 0.3 + 0.9
     off_grid[4] += 0.05
     cases = [((np.arange(3) + 1) * 0.7, 1), ((np.arange(5) + 1) * 0.5, 1), (np.arange(9) * 0.4, 1),
-             (np.arange(8) * 0.3 + 0.9, 1), (off_grid, 0)]
+             (np.arange(8) * 0.3 + 0.9, 1), (np.arange(8) * 0.3 + 0.95, 0), (off_grid, 0)]
     for rho_k, want_uni in cases:
         knots = (rho_k * rho_k).astype(np.float32)
         coefs = gpu.spline_build(-np.expm1(-knots.astype(np.float64)), knots)

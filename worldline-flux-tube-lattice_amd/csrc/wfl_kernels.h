@@ -464,7 +464,9 @@ __device__ __forceinline__ void loop_sums(const R* __restrict__ rl, int N, int M
 // two accumulations, with the same roundings as point_eval.  Physical
 // mode, n = 2, the plain node (the caller has checked that no image tube
 // of the wavefront asks for the accurate Gaussian node).  The FP32 sums
-// over one point's image tubes are added to P and S in FP64 per point.
+// over one window row's image tubes are added to P and S in FP64 (one sum
+// over all of a point's image tubes doubled the rounding of the group sums
+// on dense lattices).
 template <class R, int PROF, int SPINOR, int M>
 __device__ __forceinline__ void lattice_sums(const R* __restrict__ rl, int N, const R* ccx, const R* ccy,
                                              const LaneC<R>& lc, int kind, const GP<R>& gp, const SplineRef& sp,
@@ -484,9 +486,9 @@ __device__ __forceinline__ void lattice_sums(const R* __restrict__ rl, int N, co
         zx[a] = fma(lc.s, wx, ccx[a]);
         mx[a] = fma(lc.m8, dx4, zx[a]);
       }
-      R p = (R)0, q = (R)0;
 #pragma unroll
       for (int b = 0; b < W; ++b) {
+        R p = (R)0, q = (R)-0.0;   // -0: the first spin term's add folds away
         const R zy = fma(lc.s, wy, ccy[b]);
         const R my = fma(lc.m8, dy4, zy);
         const R zy2 = fma(zy, zy, tiny), my2 = fma(my, my, tiny);
@@ -501,9 +503,9 @@ __device__ __forceinline__ void lattice_sums(const R* __restrict__ rl, int N, co
           p = fma(E, gA, fma(X, gm, p));
           if (SPINOR) q += fA + (fk + fk);
         }
+        P[c] += (double)p;
+        if (SPINOR) S[c] += (double)q;
       }
-      P[c] += (double)p;
-      if (SPINOR) S[c] += (double)q;
     }
   }
 }

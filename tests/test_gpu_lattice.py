@@ -10,9 +10,11 @@ QED, physical mode, Simpson n = 2, lane-per-tuple and wave-per-loop
 mappings.
 
 Bars: FP64 kernels within 1e-10 of the largest per-loop value and group
-sum of the case; mixed kernels within 2e-5 per loop and 1e-5 in the group
-sums, relative to the case's largest (the single-image bars of the other
-parity tests)."""
+sum of the case; mixed kernels within 1e-6 per loop and 5e-6 in the group
+sums, relative to the case's largest (measured: at most 2.6e-7 and 2.6e-6,
+the latter for spinor QED at l = 0.1 lambda with both mappings; one FP32
+sum over a point's 25 image tubes in the lane-per-tuple inner loop gave
+7.4e-6 there)."""
 import numpy as np
 import pytest
 
@@ -72,7 +74,7 @@ def test_lattice_5x5_window(gpu, orc, loops, prec, spinor, mapping):
         if prec == 0:
             assert e1 <= 1e-10 and e2 <= 1e-10
         else:
-            assert e1 <= 2e-5 and e2 <= 1e-5
+            assert e1 <= 1e-6 and e2 <= 5e-6
 
 
 def test_interaction_energy_small_lattice_vs_oracle(gpu, orc, loops):

@@ -320,17 +320,19 @@ def test_spline_uniform_grid_rows_past_the_table(gpu, orc, loops, spinor):
         assert e1 < 2e-5 and e2 < 1e-5
 
 
+@pytest.mark.parametrize("images", [1, 2])
 @pytest.mark.parametrize("spinor", [0, 1])
-def test_spline_uniform_grid_on_a_lattice(gpu, orc, loops, spinor):
+def test_spline_uniform_grid_on_a_lattice(gpu, orc, loops, spinor, images):
     """A uniform-grid spline tube on a square lattice (spacing 1.5, 3 x 3
-    image window): the rows past the last knot must reach the farthest image
-    tube of the window plus the loops' extent, which the plan's bound takes
-    from the window; x_cm off the axes inside the cell.  Mixed physical mode
-    (the direct-index kernels) and FP64 (the exact search), against the
-    oracle."""
+    and 5 x 5 image windows, whose tubes the mixed kernels sum in the inner
+    loop): the rows past the last knot must reach the farthest image tube of
+    the window plus the loops' extent, which the plan's bound takes from the
+    window; x_cm off the axes inside the cell.  Mixed physical mode (the
+    direct-index kernels) and FP64 (the exact search), against the oracle."""
     knots, coefs = gaussian_spline(gpu, n=128, rho_max=6.0, l2=0.25)
-    geoms = [gpu.Geometry(kind=gpu.SPLINE, F=1.0, l2=0.25, a=1.0, lattice=1.5, images=1, spline_table=0)]
-    og = [orc.Geom(orc.SPLINE, F=1.0, l2=0.25, tubedist=1.0, lattice=1.5, images=1, knots=knots, coefs=coefs)]
+    geoms = [gpu.Geometry(kind=gpu.SPLINE, F=1.0, l2=0.25, a=1.0, lattice=1.5, images=images, spline_table=0)]
+    og = [orc.Geom(orc.SPLINE, F=1.0, l2=0.25, tubedist=1.0, lattice=1.5, images=images, knots=knots,
+                   coefs=coefs)]
     tg, tcx, tcy, tT = tuples([0.1, 0.6, 1.2], [0.01, 0.2, 1.5])
     tcy = 0.4 * tcx + 0.05
     for prec, bars in ((gpu.MIXED, (2e-5, 1e-5)), (gpu.FP64, (1e-10, 1e-10))):

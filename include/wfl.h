@@ -166,11 +166,30 @@ int wfl_plan_ngroups(const wfl_plan *p);
  * are small (sigma sqrt(T) < 1/2, sigma^2 = log2(e)/l2) and can reach the tube
  * core, where the plain quotient (1 - 2^-u)/u loses relative accuracy. */
 int wfl_plan_accurate_tuples(const wfl_plan *p);
-/* 1 when a mixed, physical-mode, spline-only plan evaluates its tables by
- * direct indexing: every table's knots lie on a uniform grid in
+/* Nonzero when a mixed, physical-mode, spline-only plan evaluates its tables
+ * by direct indexing: every table's knots lie on a uniform grid in
  * r = sqrt(rho^2) (the layout of periodic.c:129-151), checked to 1e-4 of
- * the spacing; 0 when the kernels use the exact interval search. */
+ * the spacing; 0 when the kernels use the exact interval search.  2 when the
+ * rows hold their cubics expanded about u = 0 (one FMA per Horner step and
+ * no expansion point per node), taken when every table is well conditioned
+ * in that form (see wfl_spline_uniform_form), 1 when each row is expanded
+ * about its own start. */
 int wfl_plan_uniform_splines(const wfl_plan *p);
+/* The form wfl_plan_uniform_splines would report for a plan over one table
+ * (n knots rho2, coefficients as wfl_spline_build writes them) whose nodes
+ * reach u = rho^2 <= umax: 0, 1 or 2.  The rows are expanded about u = 0
+ * when, on every row, sum_k |d_k| u^k <= 16 |P(u)| and
+ * sum_k k |d_k| u^(k-1) <= 16 max |P'| for the row's cubic
+ * P(u) = sum_k d_k u^k.  With rows != NULL and a nonzero form, also writes
+ * up to max_rows rows of 4 floats as the kernels read them: the guard row
+ * (for sqrt(u) < h/4), then row i for sqrt(u) in [i h, (i + 1) h), with
+ * h = (sqrt(rho2[n-1]) - sqrt(rho2[0])) / (n - 1); *nrows = rows written.
+ * Form 1 rows hold the cubic in u - fl32(i h)^2 (guard row: u - (h/2)^2),
+ * form 2 rows the cubic in u.  Host only: makes no HIP call.  No
+ * counterpart in the reference (periodic.c:129-151 defines the table,
+ * intEV.cu:64-107 evaluates it). */
+int wfl_spline_uniform_form(const float *rho2, const float *coefs, int n, double umax, float *rows,
+                            int max_rows, int *nrows);
 /* 1 when the lane-per-tuple kernels of a mixed plan use the axis form:
  * physical mode, Simpson n = 2, no lattice geometry and every tuple's x_cm
  * on the x axis (cy = 0), so that only z_x = c + s w_x is formed per

@@ -104,3 +104,76 @@ def test_expansion_point_from_the_row(table):
     dx = uu - rk.astype(np.float64)[rows] ** 2
     width = (2.0 * row[rows] + 1.0) * float(h) ** 2
     assert np.all(dx > -4e-7 * uu) and np.all(dx < width * (1 + 1e-5))
+
+
+def gaussian_table(wfl, l2, n=512, rho_max=20.0):
+    rho = (np.arange(n) + 1) * (rho_max / n)
+    knots = (rho * rho).astype(np.float32)
+    return knots, wfl.spline_build(-np.expm1(-knots.astype(np.float64) / l2), knots)
+
+
+def test_origin_form_gate(wfl):
+    """Rows expanded about u = 0 (form 2) for tables well conditioned in that
+    form, rows about their start (form 1) otherwise, the exact search (0)
+    off the grid.  A Gaussian tube of lambda^2 <= 10 on the 512-knot grid to
+    rho = 20 is smooth on the scale of its rows; at lambda^2 = 100 the last
+    rows, and a linear f everywhere far out, cancel in the expansion about 0
+    (the FP32 coefficients' c3 noise times kappa^3)."""
+    for l2 in (0.01, 0.1, 1.0, 10.0):
+        assert wfl.spline_uniform_form(*gaussian_table(wfl, l2), 400.0) == 2, l2
+    assert wfl.spline_uniform_form(*gaussian_table(wfl, 100.0), 400.0) == 1
+    knots, _ = gaussian_table(wfl, 1.0)
+    assert wfl.spline_uniform_form(knots, wfl.spline_build(knots.astype(np.float64) / 3.0, knots), 400.0) == 1
+    off = knots.copy()
+    off[3] = np.float32((np.sqrt(float(off[3])) + 1e-3 * 20.0 / 512) ** 2)
+    assert wfl.spline_uniform_form(off, wfl.spline_build(-np.expm1(-off.astype(np.float64)), off), 400.0) == 0
+    # more rows than the LDS layout holds: no direct index at all
+    assert wfl.spline_uniform_form(*gaussian_table(wfl, 1.0), 1e4) == 0
+
+
+def kernel_g_origin(u, rows, inv_h):
+    """The origin-form node as the FP32 kernel forms it, FMAs exact with one
+    rounding; the reciprocal correctly rounded (the hardware's is within one
+    ulp of it)."""
+    row, _ = kernel_row(u, inv_h)
+    c = rows[row + 1]
+    rcp = fl32(1.0 / u.astype(np.float64))
+    q = fma32(u, fma32(u, c[:, 3], c[:, 2]), c[:, 1])
+    return fma32(c[:, 0], rcp, q)
+
+
+def test_origin_form_accuracy(wfl):
+    """The library's rows of form 2 (expanded about u = 0) for the 512-knot
+    lambda = 1 table, evaluated as the FP32 kernel evaluates them, against
+    the table's cubic in FP64 divided by u: the error of g at the level of
+    FP32 rounding (rms 2e-8), with no coherent offset over ranges of rows
+    (mean within 2e-9 on each of [k0, 1), [1, 4), [4, 16), [16, 64)), which
+    the host's compensation of the coefficient roundings and its spreading
+    of d0's rounding over rows are for: plain rounding of the expansion
+    about 0 leaves offsets of 1e-8.  Rows past the last knot hold the
+    constant; below the first knot, the line."""
+    if np.finfo(LD).nmant < 60:
+        pytest.skip("long double lacks the 64-bit significand the exact FMA emulation needs")
+    knots, coefs = gaussian_table(wfl, 1.0)
+    form, rows = wfl.spline_uniform_form(knots, coefs, 500.0, rows=True)
+    assert form == 2 and len(rows) > 520
+    inv_h = np.float32(512 / 20.0)
+    k = knots.astype(np.float64)
+    c = coefs.astype(np.float64).reshape(-1, 4)
+    rng = np.random.default_rng(11)
+    u = fl32(rng.uniform(0.0, 21.0, 400000) ** 2)
+    u = u[u > 1e-6]
+    u64 = u.astype(np.float64)
+    j = np.searchsorted(k, u64, side="right") - 1
+    jj = np.clip(j, 0, len(k) - 1)
+    x = u64 - k[jj]
+    f = c[jj, 0] + x * (c[jj, 1] + x * (c[jj, 2] + x * c[jj, 3]))
+    f = np.where(j < 0, u64 * c[0, 0] / k[0], np.where(j >= len(k) - 1, c[-1, 0], f))
+    ref = f / u64
+    rel = kernel_g_origin(u, rows, inv_h).astype(np.float64) / ref - 1.0
+    print(f"max {np.abs(rel).max():.2e} rms {np.sqrt((rel ** 2).mean()):.2e} mean {rel.mean():+.2e}")
+    assert np.abs(rel).max() < 4e-7 and np.sqrt((rel ** 2).mean()) < 3.5e-8
+    for lo, hi in ((float(k[0]), 1.0), (1.0, 4.0), (4.0, 16.0), (16.0, 64.0)):
+        m = (u64 >= lo) & (u64 < hi)
+        print(f"  [{lo:.4f}, {hi}): rms {np.sqrt((rel[m] ** 2).mean()):.2e} mean {rel[m].mean():+.2e}")
+        assert abs(rel[m].mean()) < 2e-9

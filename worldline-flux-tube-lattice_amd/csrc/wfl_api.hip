@@ -529,6 +529,7 @@ struct SplineBlob {
   std::vector<SplineTab> tab;
   int uni_f4 = 0;        // uniform-grid rows of all tables (a prefix of data)
   bool all_uni = false;  // every table has them
+  bool uni_origin = false;  // they hold their cubics about u = 0
 };
 
 // Knots on a uniform grid in r = sqrt(rho^2) that starts on the grid:
@@ -561,6 +562,77 @@ static double spline_uniform_rows(const SplineTab& st, double umax) {
   if (!(umax >= 0.0) || !std::isfinite(umax)) return HUGE_VAL;
   const double t = sqrt(umax * (1.0 + 1e-5)) * (double)st.uinv_h * (1.0 + 1e-6);
   return floor(t) + 3.0;
+}
+
+// The exact lookup's row (0 the line below the first knot, n the constant
+// past the last) whose cubic uniform row i holds (i = -1: the guard row).
+static int uniform_row_interval(const SplineTab& st, int i) {
+  return std::min(std::max(std::max(i, 0) - st.ulo, 0), st.n);
+}
+
+// Whether the cubics of a uniform table's rows, expanded about u = 0 as
+// P(u) = d0 + d1 u + d2 u^2 + d3 u^3, keep the accuracy of the expanded
+// rows in FP32: on every row's range of r = sqrt(u) (9 points),
+// sum_k |d_k| u^k <= 16 |P(u)| (the relative rounding error of f stays
+// within 16 times that of its value) and sum_k k |d_k| u^(k-1) <= 16 max |P'|
+// (the derivative's error within 16 roundings of its largest value).
+// Smooth profiles of a few knots' width or more pass; a table whose cubics
+// cancel about 0 (a nearly linear f far out, a wide tube's last rows) does
+// not.  rows: the table's exact-lookup rows.
+static bool spline_origin_ok(const float4* rows, const float* k, const SplineTab& st) {
+  double dmax = 0.0, smax = 0.0;
+  for (int i = -1; i < st.urows; ++i) {
+    const int r = uniform_row_interval(st, i);
+    const float4 cr = rows[r];
+    const double kr = r == 0 ? 0.0 : (double)k[r - 1];
+    const double c0 = cr.x, c1 = cr.y, c2 = cr.z, c3 = cr.w;
+    const double d3 = c3, d2 = c2 - 3.0 * c3 * kr, d1 = c1 - kr * (2.0 * c2 - 3.0 * c3 * kr);
+    const double d0 = c0 - kr * (c1 - kr * (c2 - c3 * kr));
+    const double lo = i < 0 ? 0.0 : i * (double)st.uh, hi = i < 0 ? 0.5 * st.uh : (i + 1) * (double)st.uh;
+    for (int s = 0; s <= 8; ++s) {
+      const double x = lo + (hi - lo) * (s / 8.0), u = x * x;
+      const double P = d0 + u * (d1 + u * (d2 + u * d3));
+      const double S = fabs(d0) + u * (fabs(d1) + u * (fabs(d2) + u * fabs(d3)));
+      if (!(S <= 16.0 * fabs(P))) return false;
+      dmax = std::max(dmax, fabs(d1 + u * (2.0 * d2 + 3.0 * d3 * u)));
+      smax = std::max(smax, fabs(d1) + u * (2.0 * fabs(d2) + 3.0 * fabs(d3) * u));
+    }
+  }
+  return smax <= 16.0 * dmax;
+}
+
+// FP32 coefficients (d0, d1, d2, d3) of P(u) = sum_k d_k u^k for a row
+// whose cubic is cr about kr, taken near u = uc (the row's centre): written
+// about uc (b_k, exact in double to rounding), d3 is rounded first and
+// d3 u^3 = d3 (uc + v)^3 subtracted from the b_k, then likewise d2, d1 and
+// d0, so that each rounding error is left as (b_k - d_k) v^k with
+// |v| <= the row's width instead of times u^k.  What remains is d0's
+// rounding, a constant relative offset per row of up to half an FP32 ulp of
+// d0 / P: d0 is rounded up or down so that these offsets, relative to
+// P(uc), sum to near zero over consecutive rows (carry: the running sum).
+// Exactly representable rows (the line below the first knot, the constant
+// past the last) come out unchanged, and a cubic through u = 0 (c0 = 0 about
+// kr = 0) keeps d0 = 0, so that g = d0/u + ... stays finite at the centre.
+static float4 origin_row(const float4& cr, double kr, double uc, double& carry) {
+  const double e = uc - kr;
+  const double c0 = cr.x, c1 = cr.y, c2 = cr.z, c3 = cr.w;
+  const double P = c0 + e * (c1 + e * (c2 + e * c3));
+  double b0 = P, b1 = c1 + e * (2.0 * c2 + 3.0 * c3 * e), b2 = c2 + 3.0 * c3 * e;
+  const float d3 = (float)c3;
+  b2 -= 3.0 * uc * (double)d3; b1 -= 3.0 * uc * uc * (double)d3; b0 -= uc * uc * uc * (double)d3;
+  const float d2 = (float)b2;
+  b1 -= 2.0 * uc * (double)d2; b0 -= uc * uc * (double)d2;
+  const float d1 = (float)b1;
+  b0 -= uc * (double)d1;
+  float d0 = (float)b0;
+  if (kr == 0.0 && cr.x == 0.0f) return make_float4(0.0f, d1, d2, d3);
+  if ((double)d0 != b0 && P != 0.0 && std::isfinite(b0)) {
+    const float other = (double)d0 > b0 ? nextafterf(d0, -HUGE_VALF) : nextafterf(d0, HUGE_VALF);
+    const double e0 = ((double)d0 - b0) / fabs(P), e1 = ((double)other - b0) / fabs(P);
+    if (fabs(carry + e1) < fabs(carry + e0)) d0 = other;
+    carry += ((double)d0 - b0) / fabs(P);
+  }
+  return make_float4(d0, d1, d2, d3);
 }
 
 // largest j <= n - 2 with k[j] <= v, 0 if none
@@ -661,23 +733,6 @@ static void build_spline_blob(int ntab, const float* const* rho2, const float* c
       d[st.cf + nt] = make_float4(c[4 * (nt - 1)], 0.0f, 0.0f, 0.0f);
     }
     sb.all_uni = sb.all_uni && st.uni;
-    if (st.uni) {
-      // row i from the exact lookup's row r (expanded about its left knot:
-      // 0 for the line, k[r - 1] otherwise) shifted exactly, in double, to
-      // kappa_i = fl32(i h)^2, what the kernel forms from the row index
-      // (i = -1/2 for the guard row, the kernel's row for sqrt(u) inv_h < 1/4)
-      float4* u = sb.data.data() + st.ucf;
-      for (int i = -1; i < st.urows; ++i) {
-        const int r = std::min(std::max(std::max(i, 0) - st.ulo, 0), nt);
-        const float4 cr = d[st.cf + r];
-        const double kr = r == 0 ? 0.0 : (double)k[r - 1];
-        const double ri = (double)(float)((i < 0 ? -0.5 : (double)i) * (double)st.uh);
-        const double e = ri * ri - kr;
-        const double c0 = cr.x, c1 = cr.y, c2 = cr.z, c3 = cr.w;
-        u[i] = make_float4((float)(c0 + e * (c1 + e * (c2 + e * c3))), (float)(c1 + e * (2.0 * c2 + 3.0 * c3 * e)),
-                           (float)(c2 + 3.0 * c3 * e), (float)c3);
-      }
-    }
     for (int i = 0; i < nt; ++i) kn[st.kn + i] = k[i];
     for (int b = 0; b < st.nb; ++b) {
       const int s0 = starts[t][b];
@@ -688,6 +743,38 @@ static void build_spline_blob(int ntab, const float* const* rho2, const float* c
     st.cf += uni_f4;                       // float4 offsets in the blob
     st.rec += uni_f4 + rows;
     st.kn += 4 * (uni_f4 + rows + btot);   // float offset in the blob
+  }
+  // Row i of a uniform table is the exact lookup's row r (expanded about its
+  // left knot: 0 for the line, k[r - 1] otherwise) shifted exactly, in
+  // double, to kappa_i = fl32(i h)^2, what the kernel forms from the row
+  // index (i = -1/2 for the guard row, the kernel's row for
+  // sqrt(u) inv_h < 1/4), or to u = 0 when every uniform table is well
+  // conditioned in that form.
+  sb.uni_origin = true;
+  for (int t = 0; t < ntab; ++t)
+    if (sb.tab[t].uni && !spline_origin_ok(sb.data.data() + sb.tab[t].cf, rho2[t], sb.tab[t])) sb.uni_origin = false;
+  for (int t = 0; t < ntab; ++t) {
+    const SplineTab& st = sb.tab[t];
+    if (!st.uni) continue;
+    const float* k = rho2[t];
+    float4* u = sb.data.data() + st.ucf;
+    double carry = 0.0;
+    for (int i = -1; i < st.urows; ++i) {
+      const int r = uniform_row_interval(st, i);
+      const float4 cr = sb.data[st.cf + r];
+      const double kr = r == 0 ? 0.0 : (double)k[r - 1];
+      if (sb.uni_origin) {
+        // the row's centre in r: (h/8)^2 for the guard row (r < h/4)
+        const double rc = (i < 0 ? 0.125 : i + 0.5) * (double)st.uh;
+        u[i] = origin_row(cr, kr, rc * rc, carry);
+        continue;
+      }
+      const double ri = (double)(float)((i < 0 ? -0.5 : (double)i) * (double)st.uh);
+      const double e = ri * ri - kr;
+      const double c0 = cr.x, c1 = cr.y, c2 = cr.z, c3 = cr.w;
+      u[i] = make_float4((float)(c0 + e * (c1 + e * (c2 + e * c3))), (float)(c1 + e * (2.0 * c2 + 3.0 * c3 * e)),
+                         (float)(c2 + 3.0 * c3 * e), (float)c3);
+    }
   }
 }
 
@@ -843,6 +930,7 @@ struct wfl_plan_s {
   float4* d_sblob = nullptr;     // spline blob
   int sblob_f4 = 0, lds_splines = 0;
   int uni_splines = 0, sblob_uni_f4 = 0;   // uniform-grid spline rows (FP32 physical)
+  int uni_origin = 0;                      // those rows expanded about u = 0
   int axis = 0;                            // axis-form records (see wfl_plan_axis_form)
   bool has_local = false;                  // some tuples take the local Taylor node (lane-per-tuple kernels)
   double* d_gsum = nullptr;
@@ -1068,6 +1156,7 @@ static int plan_create_impl(wfl_plan** out, const wfl_batch* b, wfl_loops* L, ch
   // FP32 physical spline-only plans on uniform-grid tables: direct-index rows in LDS
   p->uni_splines = all_spline && sblob.all_uni && (size_t)sblob.uni_f4 * sizeof(float4) <= kSplineLdsBytes;
   p->sblob_uni_f4 = sblob.uni_f4;
+  p->uni_origin = p->uni_splines && sblob.uni_origin;
   if (p->uni_splines) p->lds_splines = 1;
   // tuples: FP32 kernel-unit values and the effective doubles
   p->tups.resize(b->ntup);
@@ -1211,7 +1300,7 @@ static void fill_launch(const wfl_plan* p, WflLaunch& a, hipStream_t s) {
   a.lanes = p->d_lanes; a.tasks = p->d_tasks; a.ntasks = p->ntasks;
   a.nslots = p->mapping == 1 ? p->ngroups * p->cpg : p->ngroups;
   a.sblob = p->d_sblob; a.sblob_f4 = p->sblob_f4; a.lds_splines = p->lds_splines;
-  a.uni_splines = p->uni_splines; a.sblob_uni_f4 = p->sblob_uni_f4;
+  a.uni_splines = p->uni_splines; a.sblob_uni_f4 = p->sblob_uni_f4; a.uni_origin = p->uni_origin;
   a.axis = p->axis; a.rec_axis = p->L->reca;
   a.linfo = p->L->info;
   a.prof_gauss = p->prof_gauss; a.spinor = p->b.spinor ? 1 : 0; a.mode = p->b.mode; a.nsub = p->b.nsub;
@@ -1338,7 +1427,7 @@ extern "C" int wfl_plan_autotune(wfl_plan* p, void* stream, int* chosen) {
   memset(&key, 0, sizeof key);
   key.ntup = p->b.ntup; key.Nl = p->L->Nl; key.N = p->L->N; key.precision = p->b.precision;
   key.spinor = p->b.spinor; key.mode = p->b.mode; key.nsub = p->b.nsub; key.prof_gauss = p->prof_gauss;
-  key.lds_splines = p->lds_splines + 2 * p->uni_splines + 4 * p->axis;
+  key.lds_splines = p->lds_splines + 2 * p->uni_splines + 4 * p->axis + 8 * p->uni_origin;
   key.ntasks1 = p->ntasks1;
   {
     bool hit = false;
@@ -1481,7 +1570,28 @@ extern "C" int wfl_plan_accurate_tuples(const wfl_plan* p) {
   for (const DevTuple& t : p->tups) n += t.acc ? 1 : 0;
   return n;
 }
-extern "C" int wfl_plan_uniform_splines(const wfl_plan* p) { return p ? p->uni_splines : 0; }
+extern "C" int wfl_plan_uniform_splines(const wfl_plan* p) {
+  return p && p->uni_splines ? (p->uni_origin ? 2 : 1) : 0;
+}
+
+extern "C" int wfl_spline_uniform_form(const float* rho2, const float* coefs, int n, double umax, float* rows,
+                                       int max_rows, int* nrows) {
+  if (nrows) *nrows = 0;
+  if (!rho2 || !coefs || n < 3 || !(umax >= 0.0)) return 0;
+  SplineBlob sb;
+  const float* k[1] = {rho2};
+  const float* c[1] = {coefs};
+  const int nn[1] = {n};
+  build_spline_blob(1, k, c, nn, WFL_MODE_PHYSICAL, sb, &umax);
+  if (!sb.all_uni) return 0;
+  if (rows && max_rows > 0) {
+    const SplineTab& st = sb.tab[0];
+    const int m = std::min(max_rows, st.urows + 1);   // the guard row, then rows 0, 1, ...
+    memcpy(rows, sb.data.data() + st.ucf - 1, sizeof(float4) * (size_t)m);
+    if (nrows) *nrows = m;
+  }
+  return sb.uni_origin ? 2 : 1;
+}
 
 extern "C" int wfl_plan_axis_form(const wfl_plan* p) { return p ? p->axis : 0; }
 extern "C" int wfl_plan_local_tuples(const wfl_plan* p) {
@@ -1742,6 +1852,7 @@ extern "C" int wfl_internal_ev_launch(const void* dev_aos, int Nl, int N, const 
     a.lds_splines = sb.tab[0].fast && sb.data.size() * sizeof(float4) <= kSplineLdsBytes;
     a.uni_splines = sb.all_uni && (size_t)sb.uni_f4 * sizeof(float4) <= kSplineLdsBytes;
     a.sblob_uni_f4 = sb.uni_f4;
+    a.uni_origin = a.uni_splines && sb.uni_origin;
     if (a.uni_splines) a.lds_splines = 1;
   }
   a.prof_gauss = g->kind == WFL_GAUSSIAN; a.spinor = fermion == 1; a.mode = mode; a.nsub = nsub;

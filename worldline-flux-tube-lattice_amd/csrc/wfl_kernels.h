@@ -305,7 +305,7 @@ __device__ __forceinline__ void finalize_loop(const double P[3], const double S[
 // (all threads, before any work); other kernels read it from global memory.
 template <int PROF>
 __device__ __forceinline__ const float4* stage_splines(const float4* __restrict__ sblob, int sblob_f4) {
-  if constexpr (PROF == P_SPLINE || PROF == P_SPLINE_U) {
+  if constexpr (PROF == P_SPLINE || uniform_spline(PROF)) {
     extern __shared__ float4 wfl_lds_splines[];
     for (int i = threadIdx.x; i < sblob_f4; i += blockDim.x) wfl_lds_splines[i] = sblob[i];
     __syncthreads();
@@ -805,6 +805,7 @@ struct WflLaunch {
   const float4* sblob; int sblob_f4;          // spline blob and its size in float4
   int lds_splines;                            // 1: spline-only kernels with the blob staged in LDS
   int uni_splines, sblob_uni_f4;              // 1: uniform-grid rows only (FP32 physical); their size
+  int uni_origin;                             // 1: those rows are expanded about u = 0
   int prof_gauss, spinor, mode, nsub, split;
   int wpb, persistent, grid;   // wavefronts per block; persistent grid size in blocks
   int* queue;

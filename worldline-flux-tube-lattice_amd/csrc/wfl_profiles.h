@@ -20,9 +20,12 @@ enum { M_REFERENCE = 0, M_PHYSICAL = 1 };
 enum { M_AXIS = 2 };
 // kernel tag for "profile chosen at run time from DevGeom::kind"
 enum { P_ANY = 99 };
-// Kernel-only profile: a spline whose knots lie on a uniform grid in
-// r = sqrt(rho^2), evaluated in FP32 physical mode by direct indexing.
-enum { P_SPLINE_U = 98 };
+// Kernel-only profiles: a spline whose knots lie on a uniform grid in
+// r = sqrt(rho^2), evaluated in FP32 physical mode by direct indexing, with
+// each row's cubic expanded about the row's start (P_SPLINE_U) or about
+// u = 0 (P_SPLINE_U0, for tables where that form loses no accuracy).
+enum { P_SPLINE_U = 98, P_SPLINE_U0 = 97 };
+constexpr bool uniform_spline(int prof) { return prof == P_SPLINE_U || prof == P_SPLINE_U0; }
 // Uniform-grid spline rows in LDS: the rows of every table from byte 0, at
 // most kUniRows rows of 16 B.
 constexpr int kUniRows = 1024;
@@ -376,11 +379,25 @@ __device__ __forceinline__ SplineRef spline_ref(const float4* blob, const DevGeo
 // two continuous derivatives and the line with a continuous value, so the
 // difference is at the FP32 rounding level.  u > 0 always holds here (the
 // FP32 kernels add 1e-30).
+// ORIGIN: the rows hold their cubics P(u) = d0 + u (d1 + u (d2 + u d3))
+// about u = 0 instead, and the node forms g = d0 (1/u) + (d1 + u (d2 + u d3))
+// directly: no expansion point and no final product, three instructions
+// fewer per node.  The host takes this form only for tables where it is
+// well conditioned: over every row, sum_k |d_k| u^k <= 16 |P(u)| and
+// sum_k k |d_k| u^(k-1) <= 16 max |P'|; it rounds the d_k to FP32 so that
+// each rounding error is carried by the lower coefficients about the row's
+// centre, and spreads d0's rounding over consecutive rows with alternating
+// signs.
+template <bool ORIGIN>
 __device__ __forceinline__ float spline_u_g(float u, const SplineRef& sp, bool need_fp, float& fp) {
   typedef const __attribute__((address_space(3))) float4 LdsRow;
   const float t = fmaf(__builtin_amdgcn_sqrtf(u), sp.uinv_h, 8388607.5f);
   const float4* row = (const float4*)(LdsRow*)(uintptr_t)((__float_as_uint(t) << 4) + sp.ubias);
   const float4 c = row[0];
+  if constexpr (ORIGIN) {
+    fp = need_fp ? fmaf(u, fmaf(u, 3.0f * c.w, 2.0f * c.z), c.y) : 0.0f;
+    return fmaf(c.x, __builtin_amdgcn_rcpf(u), fmaf(u, fmaf(u, c.w, c.z), c.y));
+  }
   const float rk = fmaf(t, sp.uh, sp.uhoff);
   const float dx = fmaf(-rk, rk, u);
   const float f = fmaf(dx, fmaf(dx, fmaf(dx, c.w, c.z), c.y), c.x);
@@ -486,7 +503,7 @@ __device__ void profile_eval(int kind, const GP<R>& g, R u, int mode, bool need_
 template <class R, int PROF>
 __device__ __forceinline__ R generic_g(int kind_rt, const GP<R>& g, R u, int mode, bool need_fp,
                                        const SplineRef& sp, R& fp) {
-  if constexpr (PROF == P_SPLINE_U && sizeof(R) == 4) return spline_u_g(u, sp, need_fp, fp);
+  if constexpr (uniform_spline(PROF) && sizeof(R) == 4) return spline_u_g<PROF == P_SPLINE_U0>(u, sp, need_fp, fp);
   const int kind = PROF == P_ANY ? kind_rt : PROF;
   R f;
   if (mode == M_PHYSICAL) {

@@ -80,7 +80,7 @@ EXPORTS = [
     "wfl_loops_destroy", "wfl_gen_loops", "wfl_read_text_loops",
     "wfl_plan_create", "wfl_plan_run", "wfl_plan_run_timed", "wfl_plan_autotune", "wfl_plan_config",
     "wfl_plan_set_config", "wfl_plan_graph_run", "wfl_plan_read", "wfl_plan_ngroups", "wfl_plan_accurate_tuples",
-    "wfl_plan_uniform_splines", "wfl_autotune_nsub", "wfl_plan_axis_form", "wfl_plan_local_tuples",
+    "wfl_plan_uniform_splines", "wfl_spline_uniform_form", "wfl_autotune_nsub", "wfl_plan_axis_form", "wfl_plan_local_tuples",
     "wfl_local_taylor",
     "wfl_plan_gsum_device", "wfl_plan_nonfinite", "wfl_plan_mapping", "wfl_plan_destroy",
     "wfl_eval_batch", "wfl_eval_batch_noalloc", "wfl_group_ev_minus1", "wfl_spline_build", "wfl_set_profile",
@@ -127,6 +127,7 @@ def _load():
         "wfl_plan_ngroups": (I, [P]),
         "wfl_plan_accurate_tuples": (I, [P]),
         "wfl_plan_uniform_splines": (I, [P]),
+        "wfl_spline_uniform_form": (I, [P, P, I, D, P, I, P]),
         "wfl_plan_axis_form": (I, [P]),
         "wfl_plan_local_tuples": (I, [P]),
         "wfl_local_taylor": (I, [D, P]),
@@ -246,6 +247,23 @@ def spline_build(fli, rho2, quirk=False):
     c4 = np.zeros((len(rho2), 4), np.float32)
     check(lib.wfl_spline_build(_ptr(fli), _ptr(rho2), len(rho2), int(quirk), _ptr(c4)))
     return c4
+
+
+def spline_uniform_form(rho2, coefs, umax, rows=False):
+    """The form of the direct-index rows a mixed physical-mode plan over this
+    one table would use for nodes up to u = umax: 0 none (exact search), 1
+    rows expanded about their start, 2 rows expanded about u = 0.  With
+    rows=True, (form, rows) with rows[0] the guard row and rows[1 + i] row i
+    (float32, shape (nrows, 4))."""
+    rho2 = np.ascontiguousarray(rho2, np.float32)
+    coefs = np.ascontiguousarray(coefs, np.float32).reshape(-1)
+    if not rows:
+        return lib.wfl_spline_uniform_form(_ptr(rho2), _ptr(coefs), len(rho2), float(umax), None, 0, None)
+    out = np.zeros((1025, 4), np.float32)
+    nr = C.c_int(0)
+    form = lib.wfl_spline_uniform_form(_ptr(rho2), _ptr(coefs), len(rho2), float(umax), _ptr(out), len(out),
+                                       C.byref(nr))
+    return form, out[:nr.value].copy()
 
 
 def effective_T(geom, T):

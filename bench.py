@@ -89,11 +89,12 @@ def gaussian_spline_table(wfl, l2=1.0, n=512, rho_max=20.0):
     return knots, wfl.spline_build(fli, knots)
 
 
-def config_c1(wfl):
+def config_c1(wfl, spinor=0):
     T = np.logspace(-3, 2, 64)
     rhos = [np.linspace(0.0, 3.0, 32)]
     geoms = [dict(kind=5, F=1.0, l2=1.0, a=1.0, spline_table=0)]
-    return dict(name="scalar_spline512_1024x1024_32rho_64T", Nl=1024, N=1024, config_id=1, spinor=0, nsub=2,
+    return dict(name=f"{'spinor' if spinor else 'scalar'}_spline512_1024x1024_32rho_64T", Nl=1024, N=1024,
+                config_id=1, spinor=spinor, nsub=2,
                 geoms=geoms, rhos=rhos, Ts=[T], splines=[gaussian_spline_table(wfl)],
                 tup=tuples_grid([0], rhos, [T]))
 
@@ -776,7 +777,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "dropin", "eaction", "profiles"])
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the loop ensemble (c3, c5)")
-    ap.add_argument("--species", default="both", choices=["scalar", "spinor", "both"], help="c3; dropin, eaction: spinor or scalar")
+    ap.add_argument("--species", default="both", choices=["scalar", "spinor", "both"], help="c3; c1: spinor for the spin factor (default scalar); dropin, eaction: spinor or scalar")
     ap.add_argument("--mode", default="physical", choices=["physical", "reference"])
     ap.add_argument("--mapping", type=int, default=-1,
                     help="0 autotune, 1 lane/tuple, 2 wave/loop, 3 lane/tuple split (default: 3 for c2, 0 otherwise)")
@@ -831,7 +832,7 @@ def main():
         for sp in {"scalar": [0], "spinor": [1], "both": [0, 1]}[args.species]:
             run_config(wfl, args, mode, config_c3(args.scale, sp))
         return
-    cfg = {"c1": lambda: config_c1(wfl), "c4": config_c4, "c5": lambda: config_c5(args.scale)}[args.config]()
+    cfg = {"c1": lambda: config_c1(wfl, int(args.species == "spinor")), "c4": config_c4, "c5": lambda: config_c5(args.scale)}[args.config]()
     run_config(wfl, args, mode, cfg)
 
 

@@ -177,10 +177,12 @@ int wfl_plan_accurate_tuples(const wfl_plan *p);
 int wfl_plan_uniform_splines(const wfl_plan *p);
 /* The form wfl_plan_uniform_splines would report for a plan over one table
  * (n knots rho2, coefficients as wfl_spline_build writes them) whose nodes
- * reach u = rho^2 <= umax: 0, 1 or 2.  The rows are expanded about u = 0
- * when, on every row, sum_k |d_k| u^k <= 16 |P(u)| and
- * sum_k k |d_k| u^(k-1) <= 16 max |P'| for the row's cubic
- * P(u) = sum_k d_k u^k.  With rows != NULL and a nonzero form, also writes
+ * reach u = rho^2 <= umax, scalar (spinor = 0) or spinor: 0, 1 or 2.  The
+ * rows are expanded about u = 0 when, at every point of every row,
+ * sum_k |d_k| u^k <= 16 |P(u)| for the row's cubic P(u) = sum_k d_k u^k,
+ * and, for spinor plans, sum_k k |d_k| u^(k-1) <= 16 |P'(u)| (the spin
+ * factor's f'; Gaussian tubes fail it where f' is small, far out).  With
+ * rows != NULL and a nonzero form, also writes
  * up to max_rows rows of 4 floats as the kernels read them: the guard row
  * (for sqrt(u) < h/4), then row i for sqrt(u) in [i h, (i + 1) h), with
  * h = (sqrt(rho2[n-1]) - sqrt(rho2[0])) / (n - 1); *nrows = rows written.
@@ -188,7 +190,7 @@ int wfl_plan_uniform_splines(const wfl_plan *p);
  * form 2 rows the cubic in u.  Host only: makes no HIP call.  No
  * counterpart in the reference (periodic.c:129-151 defines the table,
  * intEV.cu:64-107 evaluates it). */
-int wfl_spline_uniform_form(const float *rho2, const float *coefs, int n, double umax, float *rows,
+int wfl_spline_uniform_form(const float *rho2, const float *coefs, int n, double umax, int spinor, float *rows,
                             int max_rows, int *nrows);
 /* 1 when the lane-per-tuple kernels of a mixed plan use the axis form:
  * physical mode, Simpson n = 2, no lattice geometry and every tuple's x_cm

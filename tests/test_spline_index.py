@@ -118,9 +118,12 @@ def test_origin_form_gate(wfl):
     off the grid.  A Gaussian tube of lambda^2 <= 10 on the 512-knot grid to
     rho = 20 is smooth on the scale of its rows; at lambda^2 = 100 the last
     rows, and a linear f everywhere far out, cancel in the expansion about 0
-    (the FP32 coefficients' c3 noise times kappa^3)."""
+    (the FP32 coefficients' c3 noise times kappa^3).  Spinor plans also need
+    f' well conditioned at every u, which a Gaussian's f' ~ e^-u is not far
+    out (against d1 ~ e^-u (1 + u + u^2/2)): form 1."""
     for l2 in (0.01, 0.1, 1.0, 10.0):
         assert wfl.spline_uniform_form(*gaussian_table(wfl, l2), 400.0) == 2, l2
+        assert wfl.spline_uniform_form(*gaussian_table(wfl, l2), 400.0, spinor=1) == 1, l2
     assert wfl.spline_uniform_form(*gaussian_table(wfl, 100.0), 400.0) == 1
     knots, _ = gaussian_table(wfl, 1.0)
     assert wfl.spline_uniform_form(knots, wfl.spline_build(knots.astype(np.float64) / 3.0, knots), 400.0) == 1

@@ -574,13 +574,16 @@ static int uniform_row_interval(const SplineTab& st, int i) {
 // P(u) = d0 + d1 u + d2 u^2 + d3 u^3, keep the accuracy of the expanded
 // rows in FP32: on every row's range of r = sqrt(u) (9 points),
 // sum_k |d_k| u^k <= 16 |P(u)| (the relative rounding error of f stays
-// within 16 times that of its value) and sum_k k |d_k| u^(k-1) <= 16 max |P'|
-// (the derivative's error within 16 roundings of its largest value).
-// Smooth profiles of a few knots' width or more pass; a table whose cubics
-// cancel about 0 (a nearly linear f far out, a wide tube's last rows) does
-// not.  rows: the table's exact-lookup rows.
-static bool spline_origin_ok(const float4* rows, const float* k, const SplineTab& st) {
-  double dmax = 0.0, smax = 0.0;
+// within 16 times that of its value), and for spin factors (spin) also
+// sum_k k |d_k| u^(k-1) <= 16 |P'(u)|, pointwise: the spin sum of a small
+// loop takes f' at one u, and the spinor weight cancels it against the
+// phase at leading order, so a relative error there is not averaged away.
+// Smooth profiles of a few knots' width or more pass the first; a table
+// whose cubics cancel about 0 (a nearly linear f far out, a wide tube's last
+// rows) does not.  A Gaussian tube fails the second wherever f' ~ e^-u is
+// small against d1 ~ e^-u (1 + u + u^2/2): spinor plans over such tables
+// keep the rows about their start.  rows: the table's exact-lookup rows.
+static bool spline_origin_ok(const float4* rows, const float* k, const SplineTab& st, bool spin) {
   for (int i = -1; i < st.urows; ++i) {
     const int r = uniform_row_interval(st, i);
     const float4 cr = rows[r];
@@ -594,11 +597,13 @@ static bool spline_origin_ok(const float4* rows, const float* k, const SplineTab
       const double P = d0 + u * (d1 + u * (d2 + u * d3));
       const double S = fabs(d0) + u * (fabs(d1) + u * (fabs(d2) + u * fabs(d3)));
       if (!(S <= 16.0 * fabs(P))) return false;
-      dmax = std::max(dmax, fabs(d1 + u * (2.0 * d2 + 3.0 * d3 * u)));
-      smax = std::max(smax, fabs(d1) + u * (2.0 * fabs(d2) + 3.0 * fabs(d3) * u));
+      if (!spin) continue;
+      const double P1 = d1 + u * (2.0 * d2 + 3.0 * d3 * u);
+      const double S1 = fabs(d1) + u * (2.0 * fabs(d2) + 3.0 * fabs(d3) * u);
+      if (!(S1 <= 16.0 * fabs(P1))) return false;
     }
   }
-  return smax <= 16.0 * dmax;
+  return true;
 }
 
 // FP32 coefficients (d0, d1, d2, d3) of P(u) = sum_k d_k u^k for a row
@@ -606,33 +611,37 @@ static bool spline_origin_ok(const float4* rows, const float* k, const SplineTab
 // about uc (b_k, exact in double to rounding), d3 is rounded first and
 // d3 u^3 = d3 (uc + v)^3 subtracted from the b_k, then likewise d2, d1 and
 // d0, so that each rounding error is left as (b_k - d_k) v^k with
-// |v| <= the row's width instead of times u^k.  What remains is d0's
-// rounding, a constant relative offset per row of up to half an FP32 ulp of
-// d0 / P: d0 is rounded up or down so that these offsets, relative to
-// P(uc), sum to near zero over consecutive rows (carry: the running sum).
+// |v| <= the row's width instead of times u^k.  What remains are two
+// constant offsets per row: d1's rounding in f' = P', and d0's in P (half
+// an FP32 ulp of d1 / P' and of d0 / P at most).  Each is rounded up or
+// down so that these offsets, relative to P'(uc) and P(uc), sum to near
+// zero over consecutive rows (carry[1], carry[0]: the running sums); d1's
+// choice is compensated in d0 before d0 is rounded.
 // Exactly representable rows (the line below the first knot, the constant
 // past the last) come out unchanged, and a cubic through u = 0 (c0 = 0 about
 // kr = 0) keeps d0 = 0, so that g = d0/u + ... stays finite at the centre.
-static float4 origin_row(const float4& cr, double kr, double uc, double& carry) {
+static float round_spread(double b, double scale, double& carry) {
+  float d = (float)b;
+  if ((double)d == b || !(scale > 0.0) || !std::isfinite(b)) return d;
+  const float other = (double)d > b ? nextafterf(d, -HUGE_VALF) : nextafterf(d, HUGE_VALF);
+  if (fabs(carry + ((double)other - b) / scale) < fabs(carry + ((double)d - b) / scale)) d = other;
+  carry += ((double)d - b) / scale;
+  return d;
+}
+static float4 origin_row(const float4& cr, double kr, double uc, double carry[2]) {
   const double e = uc - kr;
   const double c0 = cr.x, c1 = cr.y, c2 = cr.z, c3 = cr.w;
   const double P = c0 + e * (c1 + e * (c2 + e * c3));
   double b0 = P, b1 = c1 + e * (2.0 * c2 + 3.0 * c3 * e), b2 = c2 + 3.0 * c3 * e;
+  const double P1 = b1;   // P'(uc)
   const float d3 = (float)c3;
   b2 -= 3.0 * uc * (double)d3; b1 -= 3.0 * uc * uc * (double)d3; b0 -= uc * uc * uc * (double)d3;
   const float d2 = (float)b2;
   b1 -= 2.0 * uc * (double)d2; b0 -= uc * uc * (double)d2;
-  const float d1 = (float)b1;
+  const float d1 = round_spread(b1, fabs(P1), carry[1]);
   b0 -= uc * (double)d1;
-  float d0 = (float)b0;
   if (kr == 0.0 && cr.x == 0.0f) return make_float4(0.0f, d1, d2, d3);
-  if ((double)d0 != b0 && P != 0.0 && std::isfinite(b0)) {
-    const float other = (double)d0 > b0 ? nextafterf(d0, -HUGE_VALF) : nextafterf(d0, HUGE_VALF);
-    const double e0 = ((double)d0 - b0) / fabs(P), e1 = ((double)other - b0) / fabs(P);
-    if (fabs(carry + e1) < fabs(carry + e0)) d0 = other;
-    carry += ((double)d0 - b0) / fabs(P);
-  }
-  return make_float4(d0, d1, d2, d3);
+  return make_float4(round_spread(b0, fabs(P), carry[0]), d1, d2, d3);
 }
 
 // largest j <= n - 2 with k[j] <= v, 0 if none
@@ -661,9 +670,10 @@ static bool spline_buckets(const float* k, int n, int nb, float r0, float inv_w,
 }
 
 // umax[t] (FP32 physical mode): the largest u = rho^2 that a node of table
-// t can see (table t gets uniform rows only if it is given and they fit).
+// t can see (table t gets uniform rows only if it is given and they fit);
+// spin: the kernels take f' as well (spinor).
 static void build_spline_blob(int ntab, const float* const* rho2, const float* const* coefs, const int* n, int mode,
-                              SplineBlob& sb, const double* umax = nullptr) {
+                              SplineBlob& sb, const double* umax = nullptr, bool spin = false) {
   sb = SplineBlob();
   int rows = 0, btot = 0, ktot = 0, urows = 0;
   std::vector<std::vector<int>> starts(ntab);
@@ -752,13 +762,14 @@ static void build_spline_blob(int ntab, const float* const* rho2, const float* c
   // conditioned in that form.
   sb.uni_origin = true;
   for (int t = 0; t < ntab; ++t)
-    if (sb.tab[t].uni && !spline_origin_ok(sb.data.data() + sb.tab[t].cf, rho2[t], sb.tab[t])) sb.uni_origin = false;
+    if (sb.tab[t].uni && !spline_origin_ok(sb.data.data() + sb.tab[t].cf, rho2[t], sb.tab[t], spin))
+      sb.uni_origin = false;
   for (int t = 0; t < ntab; ++t) {
     const SplineTab& st = sb.tab[t];
     if (!st.uni) continue;
     const float* k = rho2[t];
     float4* u = sb.data.data() + st.ucf;
-    double carry = 0.0;
+    double carry[2] = {0.0, 0.0};
     for (int i = -1; i < st.urows; ++i) {
       const int r = uniform_row_interval(st, i);
       const float4 cr = sb.data[st.cf + r];
@@ -1137,7 +1148,7 @@ static int plan_create_impl(wfl_plan** out, const wfl_batch* b, wfl_loops* L, ch
       umax[tab] = std::max(umax[tab], node_u_bound(G, b->tup_cx[t], b->tup_cy[t], sqrt(b->tup_T[t]), rmax));
     }
     build_spline_blob(b->nspline, kp.data(), cp.data(), np.data(), b->mode, sblob,
-                      b->precision == WFL_MIXED ? umax.data() : nullptr);
+                      b->precision == WFL_MIXED ? umax.data() : nullptr, b->spinor != 0);
   }
   std::vector<DevGeom> geoms(b->ngeom);
   p->prof_gauss = 1;
@@ -1574,15 +1585,15 @@ extern "C" int wfl_plan_uniform_splines(const wfl_plan* p) {
   return p && p->uni_splines ? (p->uni_origin ? 2 : 1) : 0;
 }
 
-extern "C" int wfl_spline_uniform_form(const float* rho2, const float* coefs, int n, double umax, float* rows,
-                                       int max_rows, int* nrows) {
+extern "C" int wfl_spline_uniform_form(const float* rho2, const float* coefs, int n, double umax, int spinor,
+                                       float* rows, int max_rows, int* nrows) {
   if (nrows) *nrows = 0;
   if (!rho2 || !coefs || n < 3 || !(umax >= 0.0)) return 0;
   SplineBlob sb;
   const float* k[1] = {rho2};
   const float* c[1] = {coefs};
   const int nn[1] = {n};
-  build_spline_blob(1, k, c, nn, WFL_MODE_PHYSICAL, sb, &umax);
+  build_spline_blob(1, k, c, nn, WFL_MODE_PHYSICAL, sb, &umax, spinor != 0);
   if (!sb.all_uni) return 0;
   if (rows && max_rows > 0) {
     const SplineTab& st = sb.tab[0];
@@ -1820,7 +1831,7 @@ extern "C" int wfl_internal_ev_launch(const void* dev_aos, int Nl, int N, const 
       if (rc) return rc;
       umax = node_u_bound(gg, (double)cx, (double)cy, (double)rtT, rmax);
     }
-    build_spline_blob(1, &kp, &cp, &nspline, mode, sb, precision == WFL_MIXED ? &umax : nullptr);
+    build_spline_blob(1, &kp, &cp, &nspline, mode, sb, precision == WFL_MIXED ? &umax : nullptr, fermion == 1);
     if (sb.data.size() > g_ev.sblob_cap) {
       if (g_ev.d_sblob) (void)hipFree(g_ev.d_sblob);
       g_ev.d_sblob = nullptr; g_ev.sblob_cap = 0;

@@ -394,14 +394,18 @@ __device__ __forceinline__ float spline_u_g(float u, const SplineRef& sp, bool n
   const float t = fmaf(__builtin_amdgcn_sqrtf(u), sp.uinv_h, 8388607.5f);
   const float4* row = (const float4*)(LdsRow*)(uintptr_t)((__float_as_uint(t) << 4) + sp.ubias);
   const float4 c = row[0];
+  // f' = c1 + 2 c2 x + 3 c3 x^2 = q + x (i + c3 x) from the Horner
+  // partials i = c2 + c3 x and q = c1 + x i: two FMAs
   if constexpr (ORIGIN) {
-    fp = need_fp ? fmaf(u, fmaf(u, 3.0f * c.w, 2.0f * c.z), c.y) : 0.0f;
-    return fmaf(c.x, __builtin_amdgcn_rcpf(u), fmaf(u, fmaf(u, c.w, c.z), c.y));
+    const float i = fmaf(u, c.w, c.z), q = fmaf(u, i, c.y);
+    fp = need_fp ? fmaf(u, fmaf(u, c.w, i), q) : 0.0f;
+    return fmaf(c.x, __builtin_amdgcn_rcpf(u), q);
   }
   const float rk = fmaf(t, sp.uh, sp.uhoff);
   const float dx = fmaf(-rk, rk, u);
-  const float f = fmaf(dx, fmaf(dx, fmaf(dx, c.w, c.z), c.y), c.x);
-  fp = need_fp ? fmaf(dx, fmaf(dx, 3.0f * c.w, 2.0f * c.z), c.y) : 0.0f;
+  const float i = fmaf(dx, c.w, c.z), q = fmaf(dx, i, c.y);
+  const float f = fmaf(dx, q, c.x);
+  fp = need_fp ? fmaf(dx, fmaf(dx, c.w, i), q) : 0.0f;
   return f * __builtin_amdgcn_rcpf(u);
 }
 
@@ -441,14 +445,15 @@ __device__ __forceinline__ void spline_eval(R u, const SplineRef& sp, int mode, 
   klo = lo ? (R)0 : (hi ? (R)sp.kl : klo);
   const float4 c = sp.cf[row];
   const R dx = u - klo;
-  f = (R)c.x + dx * ((R)c.y + dx * ((R)c.z + dx * (R)c.w));
+  const R in = (R)c.z + dx * (R)c.w, q = (R)c.y + dx * in;
+  f = (R)c.x + dx * q;
   fp = (R)0;
   if (need_fp) {
     if (mode == M_REFERENCE) {
       R s = Math<R>::sin(Math<R>::sqrt(u) * (R)3.14159265358979323846);
       fp = (s > (R)100) ? (R)0 : Math<R>::exp(-s * s * g.inv_l2) * g.inv_l2;
     } else {
-      fp = (R)c.y + dx * ((R)2 * (R)c.z + dx * (R)3 * (R)c.w);
+      fp = q + dx * (in + dx * (R)c.w);   // c1 + 2 c2 dx + 3 c3 dx^2
     }
   }
 }

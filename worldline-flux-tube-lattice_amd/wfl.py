@@ -127,7 +127,7 @@ def _load():
         "wfl_plan_ngroups": (I, [P]),
         "wfl_plan_accurate_tuples": (I, [P]),
         "wfl_plan_uniform_splines": (I, [P]),
-        "wfl_spline_uniform_form": (I, [P, P, I, D, P, I, P]),
+        "wfl_spline_uniform_form": (I, [P, P, I, D, I, P, I, P]),
         "wfl_plan_axis_form": (I, [P]),
         "wfl_plan_local_tuples": (I, [P]),
         "wfl_local_taylor": (I, [D, P]),
@@ -249,19 +249,21 @@ def spline_build(fli, rho2, quirk=False):
     return c4
 
 
-def spline_uniform_form(rho2, coefs, umax, rows=False):
+def spline_uniform_form(rho2, coefs, umax, spinor=0, rows=False):
     """The form of the direct-index rows a mixed physical-mode plan over this
-    one table would use for nodes up to u = umax: 0 none (exact search), 1
-    rows expanded about their start, 2 rows expanded about u = 0.  With
+    one table would use for nodes up to u = umax, scalar or spinor: 0 none
+    (exact search), 1 rows expanded about their start, 2 rows expanded about
+    u = 0.  With
     rows=True, (form, rows) with rows[0] the guard row and rows[1 + i] row i
     (float32, shape (nrows, 4))."""
     rho2 = np.ascontiguousarray(rho2, np.float32)
     coefs = np.ascontiguousarray(coefs, np.float32).reshape(-1)
     if not rows:
-        return lib.wfl_spline_uniform_form(_ptr(rho2), _ptr(coefs), len(rho2), float(umax), None, 0, None)
+        return lib.wfl_spline_uniform_form(_ptr(rho2), _ptr(coefs), len(rho2), float(umax), int(spinor), None, 0,
+                                           None)
     out = np.zeros((1025, 4), np.float32)
     nr = C.c_int(0)
-    form = lib.wfl_spline_uniform_form(_ptr(rho2), _ptr(coefs), len(rho2), float(umax), _ptr(out), len(out),
+    form = lib.wfl_spline_uniform_form(_ptr(rho2), _ptr(coefs), len(rho2), float(umax), int(spinor), _ptr(out), len(out),
                                        C.byref(nr))
     return form, out[:nr.value].copy()
 

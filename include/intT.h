@@ -97,6 +97,14 @@ double PeriodicCA(double l2);
 double FixFluxCA(double l2);
 double ClassAction(double F, double l2);
 double *EAction(int Npoints, void *p);
+/* periodic.c:45-245 -- the spline profile's table (malloc'd, freed by the
+ * caller), with no GSL: f(rho) by quadrature of periodic.c:18-24's ODE,
+ * whose right-hand side does not depend on y; and intEV.cu:64-107's host
+ * lookup, which periodic.c's testspline calls (table length: Nspline). */
+float4 *flspline(double *fli, float *rho2, int N);
+float4 *getspline(float *rho2, double lambda2, double a, int Npoints);
+double testspline(float rho2, double lambda2, double a, int Npoints, float *rho2sp, float4 *flcoefs);
+float interp(float rho2, float *rho2sp, float4 *coefs);
 /* Integrand.c:20-172 closed forms */
 double Exact(double T, double B, int fermion);
 double MagField(double rho2, void *p);

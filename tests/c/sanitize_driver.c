@@ -42,6 +42,22 @@ int main(int argc, char **argv) {
   for (int i = 0; i < ns; ++i) { kn[i] = (float)((i + 1) * (i + 1) * 0.01); fli[i] = -expm1(-(double)kn[i]); }
   CHECK(wfl_spline_build(fli, kn, ns, 0, c4) == WFL_OK);
   CHECK(wfl_spline_build(fli, kn, ns, 1, c4) == WFL_OK);
+  /* periodic.c's table, flspline, the host lookup and testspline */
+  {
+    /* 512 knots: interp takes the table length from the drop-in's Nspline (512 by default) */
+    float *k2 = malloc(sizeof(float) * 512);
+    float4 *pc = getspline(k2, 0.3, 1.1, 512);
+    CHECK(pc != NULL);
+    float4 *fc = flspline(fli, kn, ns);
+    CHECK(fc != NULL);
+    if (pc) {
+      CHECK(fabs(testspline(1.0f, 0.3, 1.1, 512, k2, pc)) < 1e-2);
+      CHECK(interp(0.5f * k2[0], k2, pc) == 0.0f);   /* outside the knots: 0 */
+      CHECK(fabs(interp(1.0f, k2, pc) - pc[25].x) < 0.1f);
+    }
+    CHECK(getspline(k2, -1.0, 1.1, 512) == NULL);
+    free(pc); free(fc); free(k2);
+  }
   /* EVMean on every branch (plain, spinor, cosh guard), FP checks on */
   float4 ws[8], wf[8];
   int WL[8];

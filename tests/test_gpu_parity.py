@@ -195,6 +195,29 @@ def test_spline_profile(gpu, orc, loops, prec, tol, mode, mapping):
         assert e1 < tol and e2 < tol
 
 
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 2e-5)])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_spline_periodic_table(gpu, orc, loops, prec, tol, mode):
+    """The table that periodic.c's getspline builds for the spline profile
+    (tubes exp(-sin^2(pi t/a)/l2) in every direction, l2 = 0.3, a = 1.1, 512
+    knots to rho = 20; here by the library's getspline), scalar and spinor,
+    against the oracle.  f grows like rho^2 far out, so the expansion about
+    u = 0 cancels and the mixed physical-mode plan keeps the rows about their
+    start (form 1); reference mode and FP64 take the exact search."""
+    knots, coefs = gpu.getspline(0.3, 1.1, 512)
+    geoms = [gpu.Geometry(kind=gpu.SPLINE, F=1.0, l2=0.3, a=1.1, spline_table=0)]
+    og = [orc.Geom(orc.SPLINE, F=1.0, l2=0.3, tubedist=1.1, knots=knots, coefs=coefs)]
+    for spinor in (0, 1):
+        seen = []
+        gs, pl, rgs, rpl = run_case(gpu, orc, loops, geoms, og, mode, 2, spinor, prec, 1,
+                                    tuples([0.0, 0.45, 2.0], [0.01, 0.3, 3.0]), splines=[(knots, coefs)],
+                                    on_plan=lambda p: seen.append(p.uniform_splines))
+        e1, e2 = relerr(pl, rpl), relerr(gs, rgs)
+        print(f"  spinor {spinor}: uniform {seen[0]} perloop {e1:.2e} gsum {e2:.2e}")
+        assert seen == [1 if (prec == gpu.MIXED and mode == 1) else 0]
+        assert e1 < tol and e2 < tol / 2
+
+
 @pytest.mark.parametrize("mapping", [1, 2])
 @pytest.mark.parametrize("nsub", [2, 4])
 def test_spline_uniform_grid(gpu, orc, loops, mapping, nsub):

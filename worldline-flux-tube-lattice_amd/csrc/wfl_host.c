@@ -15,6 +15,8 @@
  *   getwl       getwl.c:6-40 (text loops into float4 AoS)
  *   wfl_gen_loops     seeded unit loops (Philox4x32-10 + Box-Muller)
  *   wfl_spline_build  periodic.c:45-110 (clamped cubic spline, zero slopes)
+ *   flspline, getspline, testspline   periodic.c:45-245 (no GSL)
+ *   interp      intEV.cu:64-107 (the host lookup periodic.c calls)
  *   wfl_logsimpson, wfl_leff_groups   T-integral on log-spaced grids
  */
 #define _GNU_SOURCE   /* feenableexcept (glibc) for the strict FP-trap mode */
@@ -824,4 +826,133 @@ int wfl_spline_build(const double *fli, const float *x, int n, int quirk, float 
   c4[4 * (n - 1) + 2] = (float)c[n - 1];
   free(w);
   return WFL_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* The spline profile's table and its host lookup (periodic.c, intEV.cu) */
+/* ------------------------------------------------------------------ */
+/* flspline (periodic.c:45-110): the coefficient rows, malloc'd, with the
+ * reference's float subtractions of the knots and float-rounded
+ * recurrence (wfl_spline_build's quirk mode). */
+float4 *flspline(double *fli, float *rho2, int N) {
+  if (!fli || !rho2 || N < 3) { wfl_set_error("flspline: need N >= 3 and buffers"); return NULL; }
+  float4 *c = (float4 *)malloc(sizeof(float4) * (size_t)N);
+  if (!c) { wfl_set_error("flspline: malloc"); return NULL; }
+  if (wfl_spline_build(fli, rho2, N, 1, (float *)c) != WFL_OK) { free(c); return NULL; }
+  return c;
+}
+
+/* f'(t) of the periodic profile as periodic.c:18-24 writes the ODE's
+ * right-hand side: (2/l2) t exp(-sin^2(pi t/a)/l2). */
+static double periodic_df(double t, double l2, double a) {
+  const double s = sin(pi * t / a);
+  return 2.0 / l2 * t * exp(-s * s / l2);
+}
+
+/* Nodes and weights of 20-point Gauss-Legendre on [-1, 1]: Newton's
+ * method on P_20 from the usual first guesses, P_20 and P_20' by the
+ * three-term recurrence. */
+static void gauss_legendre20(double x[20], double w[20]) {
+  const int n = 20;
+  for (int i = 0; i < n; ++i) {
+    double z = cos(pi * (i + 0.75) / (n + 0.5)), dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p1 = 1.0, p2 = 0.0;
+      for (int j = 1; j <= n; ++j) {
+        const double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+      }
+      dp = n * (z * p1 - p2) / (z * z - 1.0);
+      const double dz = p1 / dp;
+      z -= dz;
+      if (fabs(dz) <= 1e-16) break;
+    }
+    x[i] = z;
+    w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
+
+/* int_t0^t1 f'(t) dt.  periodic.c integrates y' = f'(t) with GSL's rk8pd;
+ * the right-hand side does not depend on y, so that ODE is this quadrature.
+ * Composite 20-point Gauss-Legendre on pieces no wider than a quarter of
+ * a sqrt(l2)/pi, over which sin^2(pi t/a)/l2 changes by at most about 3
+ * where the integrand is not negligible: error near the double rounding
+ * of the sum. */
+static double periodic_integral(double t0, double t1, double l2, double a, const double *x, const double *w) {
+  if (!(t1 > t0)) return 0.0;
+  const double piece = 0.25 * a * sqrt(l2) / pi;
+  const double m = ceil((t1 - t0) / piece);
+  const long np = m < 1.0 ? 1 : (m > 1e8 ? 100000000L : (long)m);
+  double sum = 0.0;
+  for (long k = 0; k < np; ++k) {
+    const double lo = t0 + (t1 - t0) * (double)k / (double)np, hi = t0 + (t1 - t0) * (double)(k + 1) / (double)np;
+    const double c = 0.5 * (lo + hi), r = 0.5 * (hi - lo);
+    double s = 0.0;
+    for (int i = 0; i < 20; ++i) s += w[i] * periodic_df(c + r * x[i], l2, a);
+    sum += r * s;
+  }
+  return sum;
+}
+
+/* getspline (periodic.c:112-162): knots rho_i = (i + 1) 20/N
+ * (rho2[i] = fl32(rho_i^2)), f(rho_i) = int_0^rho_i f'(t) dt, and the
+ * coefficient rows from flspline; malloc'd.  Where the reference's fixed
+ * rk8pd steps of 20/N resolve the integrand its values agree with these to
+ * its truncation error; it needs no GSL. */
+float4 *getspline(float *rho2, double lambda2, double a, int Npoints) {
+  if (!rho2 || Npoints < 3 || !(lambda2 > 0.0) || !(a > 0.0)) {
+    wfl_set_error("getspline: need Npoints >= 3, lambda2 > 0, a > 0 and a buffer");
+    return NULL;
+  }
+  double *fl = (double *)malloc(sizeof(double) * (size_t)Npoints);
+  if (!fl) { wfl_set_error("getspline: malloc"); return NULL; }
+  double x[20], w[20];
+  gauss_legendre20(x, w);
+  const double h = 20.0 / (double)Npoints;
+  double y = 0.0, rho = 0.0;
+  for (int i = 0; i < Npoints; ++i) {
+    const double next = (double)(i + 1) * h;
+    y += periodic_integral(rho, next, lambda2, a, x, w);
+    rho = next;
+    rho2[i] = (float)(rho * rho);
+    fl[i] = y;
+  }
+  float4 *c = flspline(fl, rho2, Npoints);
+  free(fl);
+  return c;
+}
+
+/* interp (intEV.cu:64-107) on the host: f at rho2 from the table in FP32,
+ * 0 outside (k0, k_{n-1}); n = the Nspline of wfl_set_profile (512 unless
+ * set). */
+static float interp_n(float rho2, const float *k, const float4 *c, int n) {
+  if (!(rho2 < k[n - 1] && rho2 > k[0])) return 0.0f;
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    const int mid = (hi + lo) / 2;
+    if (rho2 >= k[mid]) lo = mid; else hi = mid;
+  }
+  const float d = rho2 - k[lo];
+  const float inner = c[lo].z + d * c[lo].w;
+  const float mid = c[lo].y + d * inner;
+  return c[lo].x + d * mid;
+}
+float interp(float rho2, float *rho2sp, float4 *coefs) {
+  if (!rho2sp || !coefs || g_nspline < 2) return 0.0f;
+  return interp_n(rho2, rho2sp, coefs, g_nspline);
+}
+
+/* testspline (periodic.c:204-245): prints "testspline: rho2=..." and
+ * returns (f - s)/f at rho2, f = int_0^sqrt(rho2) f'(t) dt (the reference
+ * integrates with adaptive rk8pd steps at 1e-9 absolute, 1e-7 relative),
+ * s = the table's FP32 lookup over its Npoints knots. */
+double testspline(float rho2, double lambda2, double a, int Npoints, float *rho2sp, float4 *flcoefs) {
+  printf("testspline: rho2=%f\n", (double)rho2);
+  if (!rho2sp || !flcoefs || Npoints < 2 || !(lambda2 > 0.0) || !(a > 0.0) || !(rho2 > 0.0f)) return NAN;
+  double x[20], w[20];
+  gauss_legendre20(x, w);
+  const double f = periodic_integral(0.0, sqrt((double)rho2), lambda2, a, x, w);
+  const double sp = (double)interp_n(rho2, rho2sp, flcoefs, Npoints);
+  return (f - sp) / f;
 }

@@ -88,6 +88,7 @@ EXPORTS = [
     "wfl_effective_tuple",
     "wfl_leff_groups", "wfl_leff_groups_est", "wfl_group_ev_minus1_est",
     "EV", "EVMean", "Tfunc", "integrateT", "getwl", "Exact", "MagField", "smTscal", "smTferm",
+    "flspline", "getspline", "testspline", "interp",
     "rhogrand", "testvalrho", "transformRho", "integrateRhotoa", "integrateRho", "PeriodicCA", "FixFluxCA",
     "ClassAction", "EAction", "wfl_eaction", "wfl_set_verbosity",
     "wfl_cell_action_groups", "wfl_radial_action_groups", "wfl_interaction_energy",
@@ -170,6 +171,10 @@ def _load():
         "wfl_set_fp_traps": (I, [I]),
         "wfl_fp_trap_count": (C.c_longlong, []),
         "getwl": (I, [P, C.c_char_p, I, I, I]),
+        "flspline": (P, [P, P, I]),
+        "getspline": (P, [P, D, D, I]),
+        "testspline": (D, [C.c_float, D, D, I, P, P]),
+        "interp": (C.c_float, [C.c_float, P, P]),
         "Exact": (D, [D, D, I]),
         "MagField": (D, [D, P]),
         "smTscal": (D, [D, D]),
@@ -183,6 +188,9 @@ def _load():
 
 
 lib = _load()
+_libc = C.CDLL(None)
+_libc.free.restype = None
+_libc.free.argtypes = [C.c_void_p]
 
 
 def _ptr(a):
@@ -266,6 +274,45 @@ def spline_uniform_form(rho2, coefs, umax, spinor=0, rows=False):
     form = lib.wfl_spline_uniform_form(_ptr(rho2), _ptr(coefs), len(rho2), float(umax), int(spinor), _ptr(out), len(out),
                                        C.byref(nr))
     return form, out[:nr.value].copy()
+
+
+def _take_rows(ptr, n):
+    """Copy n float4 rows the library malloc'd at ptr, then free them."""
+    if not ptr:
+        raise WflError(lib.wfl_last_error().decode())
+    try:
+        return np.ctypeslib.as_array((C.c_float * (4 * n)).from_address(ptr)).reshape(n, 4).copy()
+    finally:
+        _libc.free(ptr)
+
+
+def getspline(lambda2, a, n=512):
+    """(rho2, coefs) of the periodic profile's table, as getspline
+    (periodic.c:112-162) builds it: knots (i + 1) 20/n, f by quadrature."""
+    rho2 = np.zeros(n, np.float32)
+    return rho2, _take_rows(lib.getspline(_ptr(rho2), float(lambda2), float(a), int(n)), n)
+
+
+def flspline(fli, rho2):
+    """flspline (periodic.c:45-110): rows with the reference's float arithmetic."""
+    fli = np.ascontiguousarray(fli, np.float64)
+    rho2 = np.ascontiguousarray(rho2, np.float32)
+    return _take_rows(lib.flspline(_ptr(fli), _ptr(rho2), len(rho2)), len(rho2))
+
+
+def testspline(rho2, lambda2, a, knots, coefs):
+    """testspline (periodic.c:204-245): (f - s)/f at rho2."""
+    knots = np.ascontiguousarray(knots, np.float32)
+    coefs = np.ascontiguousarray(coefs, np.float32)
+    return lib.testspline(float(rho2), float(lambda2), float(a), len(knots), _ptr(knots), _ptr(coefs))
+
+
+def interp(rho2, knots, coefs):
+    """interp (intEV.cu:64-107) on the host, over the Nspline knots that
+    set_profile last gave (512 unless set)."""
+    knots = np.ascontiguousarray(knots, np.float32)
+    coefs = np.ascontiguousarray(coefs, np.float32)
+    return lib.interp(float(rho2), _ptr(knots), _ptr(coefs))
 
 
 def effective_T(geom, T):

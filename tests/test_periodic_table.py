@@ -97,3 +97,22 @@ def test_testspline_and_interp(wfl, l2, a):
     # outside the knots the reference's lookup returns 0
     assert wfl.interp(float(knots[0]) * 0.5, knots, coefs) == 0.0
     assert wfl.interp(float(knots[-1]) * 1.01, knots, coefs) == 0.0
+
+
+def test_periodic_table_keeps_rows_about_their_start(wfl):
+    """f of the periodic table grows like rho^2 (the flux of every tube
+    within rho), nearly linear in u far out, where its cubics cancel about
+    u = 0: the direct-index rows stay expanded about their start for both
+    species (the GPU parity test on this table checks the plan's form)."""
+    knots, coefs = wfl.getspline(0.3, 1.1, 512)
+    assert wfl.spline_uniform_form(knots, coefs, 400.0) == 1
+    assert wfl.spline_uniform_form(knots, coefs, 400.0, spinor=1) == 1
+
+
+def test_table_builders_reject_bad_arguments(wfl):
+    with pytest.raises(wfl.WflError, match="flspline"):
+        wfl.flspline(np.zeros(2), np.array([0.1, 0.2], np.float32))
+    with pytest.raises(wfl.WflError, match="getspline"):
+        wfl.getspline(-1.0, 1.0, 64)
+    with pytest.raises(wfl.WflError, match="knots must increase"):
+        wfl.flspline(np.zeros(4), np.array([0.1, 0.3, 0.2, 0.4], np.float32))

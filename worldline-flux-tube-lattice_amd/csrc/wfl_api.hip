@@ -1765,10 +1765,19 @@ extern "C" int wfl_autotune_nsub(const wfl_batch* b, wfl_loops* L, double tol, i
 // ------------------------------------------------------ EV() helper
 // Loop copies of the caller's AoS device buffer, reused while the pointer
 // and shape stay the same.
+// The launch's geometry and tuple travel in one block: DevGeom at 0,
+// DevTuple at kEvTupOff; staged in pinned host memory and sent with one
+// asynchronous copy on the launch's stream.  Every EV() call waits for its
+// results (the device-to-host copy of Wsscal) before it returns, so the
+// next call may rewrite the staging block.
+static constexpr size_t kEvTupOff = (sizeof(DevGeom) + 255) / 256 * 256;
+static constexpr size_t kEvBlock = kEvTupOff + sizeof(DevTuple);
 static struct {
   const void* aos = nullptr;
   int Nl = 0, N = 0;
   wfl_loops* L = nullptr;
+  void* d_block = nullptr;       // DevGeom, then DevTuple at kEvTupOff
+  void* h_stage = nullptr;       // pinned copy source of the same layout
   DevTuple* d_tup = nullptr;
   DevGeom* d_geom = nullptr;
   unsigned long long* d_nf = nullptr;
@@ -1778,11 +1787,12 @@ static struct {
 
 extern "C" void wfl_internal_ev_release(void) {
   if (g_ev.L) wfl_loops_destroy(g_ev.L);
-  if (g_ev.d_tup) (void)hipFree(g_ev.d_tup);
-  if (g_ev.d_geom) (void)hipFree(g_ev.d_geom);
+  if (g_ev.d_block) (void)hipFree(g_ev.d_block);
+  if (g_ev.h_stage) (void)hipHostFree(g_ev.h_stage);
   if (g_ev.d_nf) (void)hipFree(g_ev.d_nf);
   if (g_ev.d_sblob) (void)hipFree(g_ev.d_sblob);
-  g_ev.L = nullptr; g_ev.d_tup = nullptr; g_ev.d_geom = nullptr; g_ev.d_nf = nullptr;
+  g_ev.L = nullptr; g_ev.d_block = nullptr; g_ev.h_stage = nullptr;
+  g_ev.d_tup = nullptr; g_ev.d_geom = nullptr; g_ev.d_nf = nullptr;
   g_ev.d_sblob = nullptr; g_ev.sblob_cap = 0;
   g_ev.aos = nullptr; g_ev.Nl = g_ev.N = 0;
 }
@@ -1799,10 +1809,13 @@ extern "C" int wfl_internal_ev_launch(const void* dev_aos, int Nl, int N, const 
     wfl_internal_ev_release();
     int rc = wfl_loops_from_aos_device(&g_ev.L, dev_aos, Nl, N);
     if (rc) return rc;
-    g_ev.aos = dev_aos; g_ev.Nl = Nl; g_ev.N = N;
-    HIP_TRY(hipMalloc(&g_ev.d_tup, sizeof(DevTuple)));
-    HIP_TRY(hipMalloc(&g_ev.d_geom, sizeof(DevGeom)));
+    HIP_TRY(hipMalloc(&g_ev.d_block, kEvBlock));
+    HIP_TRY(hipHostMalloc(&g_ev.h_stage, kEvBlock, hipHostMallocDefault));
+    g_ev.d_geom = (DevGeom*)g_ev.d_block;
+    g_ev.d_tup = (DevTuple*)((char*)g_ev.d_block + kEvTupOff);
     HIP_TRY(hipMalloc(&g_ev.d_nf, sizeof(unsigned long long)));
+    // the key last: a failure above leaves it clear, and the next call sets up again
+    g_ev.aos = dev_aos; g_ev.Nl = Nl; g_ev.N = N;
   }
   int rc = loops_records(g_ev.L, precision);
   if (rc) return rc;
@@ -1852,8 +1865,9 @@ extern "C" int wfl_internal_ev_launch(const void* dev_aos, int Nl, int N, const 
     dg.acc_rmax = rmax;
     dt.acc = tuple_needs_accurate(dg, dt, unit_scale(gg, scaled), rmax);
   }
-  HIP_TRY(hipMemcpy(g_ev.d_geom, &dg, sizeof dg, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(g_ev.d_tup, &dt, sizeof dt, hipMemcpyHostToDevice));
+  memcpy(g_ev.h_stage, &dg, sizeof dg);
+  memcpy((char*)g_ev.h_stage + kEvTupOff, &dt, sizeof dt);
+  HIP_TRY(hipMemcpyAsync(g_ev.d_block, g_ev.h_stage, kEvBlock, hipMemcpyHostToDevice, 0));
   WflLaunch a;
   memset(&a, 0, sizeof a);
   a.rec = precision == WFL_MIXED ? (const void*)g_ev.L->rec32 : (const void*)g_ev.L->rec64;

@@ -180,3 +180,46 @@ def test_origin_form_accuracy(wfl):
         m = (u64 >= lo) & (u64 < hi)
         print(f"  [{lo:.4f}, {hi}): rms {np.sqrt((rel[m] ** 2).mean()):.2e} mean {rel[m].mean():+.2e}")
         assert abs(rel[m].mean()) < 2e-9
+
+
+def test_row_start_form_accuracy(wfl):
+    """The library's rows of form 1 (each cubic expanded about its row's
+    start fl32(i h)^2, the guard row about (h/2)^2), for the lambda^2 = 100
+    table that keeps them, evaluated as the FP32 kernel does (row from the
+    FMA's bits, r = fma(t, h, -2^23 h), dx = fma(-r, r, u), Horner, times
+    the reciprocal) against the table's cubic in FP64 divided by u: within
+    FP32 rounding, no offset coherent over rows, and f' (the spinor's
+    derivative from the Horner partials) at FP32 rounding too (rms below
+    1e-7 relative) wherever it is above 1e-3 of its largest value."""
+    if np.finfo(LD).nmant < 60:
+        pytest.skip("long double lacks the 64-bit significand the exact FMA emulation needs")
+    knots, coefs = gaussian_table(wfl, 100.0)
+    form, rows = wfl.spline_uniform_form(knots, coefs, 500.0, rows=True)
+    assert form == 1 and len(rows) > 520
+    h = np.float32(20.0 / 512)
+    inv_h = np.float32(512 / 20.0)
+    k = knots.astype(np.float64)
+    c = coefs.astype(np.float64).reshape(-1, 4)
+    rng = np.random.default_rng(13)
+    u = fl32(rng.uniform(float(np.sqrt(k[0])), 19.9, 300000) ** 2)
+    u64 = u.astype(np.float64)
+    j = np.clip(np.searchsorted(k, u64, side="right") - 1, 0, len(k) - 2)
+    x = u64 - k[j]
+    f = c[j, 0] + x * (c[j, 1] + x * (c[j, 2] + x * c[j, 3]))
+    fp = c[j, 1] + x * (2.0 * c[j, 2] + 3.0 * c[j, 3] * x)
+    row, t = kernel_row(u, inv_h)
+    cc = rows[row + 1]
+    rk = fma32(t, h, np.float32(-8388608.0) * h)
+    dx = fma32(-rk, rk, u)
+    i = fma32(dx, cc[:, 3], cc[:, 2])
+    q = fma32(dx, i, cc[:, 1])
+    fk = fma32(dx, q, cc[:, 0])
+    g = (fk * fl32(1.0 / u64)).astype(np.float64)
+    rel = g / (f / u64) - 1.0
+    print(f"g: max {np.abs(rel).max():.2e} rms {np.sqrt((rel ** 2).mean()):.2e} mean {rel.mean():+.2e}")
+    assert np.abs(rel).max() < 3e-7 and abs(rel.mean()) < 2e-9
+    d = fma32(dx, fma32(dx, cc[:, 3], i), q).astype(np.float64)
+    big = np.abs(fp) > 1e-3 * np.abs(fp).max()
+    relp = d[big] / fp[big] - 1.0
+    print(f"f': max {np.abs(relp).max():.2e} rms {np.sqrt((relp ** 2).mean()):.2e}")
+    assert np.abs(relp).max() < 1e-6 and np.sqrt((relp ** 2).mean()) < 1e-7
